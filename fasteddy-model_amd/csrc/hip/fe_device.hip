@@ -1,0 +1,564 @@
+/*
+ * fe_device.hip -- the C-ABI device layer of the FastEddy time step on MI355X.
+ *
+ * Replaces the reference's CUDA modules behind their own entry points
+ * (include/fasteddy_amd.h): SRC/FECUDA/fecuda.c, SRC/GRID/CUDA/cuda_grid.c,
+ * SRC/HYDRO_CORE/CUDA/cuda_hydroCore.c and SRC/TIME_INTEGRATION/CUDA/cuda_timeInt.c.
+ * Inputs are the host modules' globals and halo-inclusive host arrays; the
+ * device keeps interior-only fields in [field][z][y][x] order plus 1-D
+ * profiles of the (flat-terrain) metric and base state.
+ *
+ * One RK3 step is three fused stage launches with out-of-place buffers:
+ *   stage 1: S0 -> S1, stage 2: (S1, S0) -> S2, stage 3: (S2, S0) -> S0,
+ * the physics stage preceded by a column pass (SGS length scale, eddy
+ * viscosity, surface layer).  A step is captured once in a hipGraph and
+ * replayed.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <vector>
+#include "../host/fe_host.h"
+#include "../../../include/fasteddy_amd.h"
+#include "fe_kernels.h"
+#include "fe_stage.hip"
+
+using namespace fe;
+
+#define HIPCHK(x)                                                                               \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) {                                                                     \
+      fprintf(stderr, "HIP error %s at %s:%d: %s\n", hipGetErrorName(e_), __FILE__, __LINE__,  \
+              hipGetErrorString(e_));                                                           \
+      return FE_ERR_HIP;                                                                        \
+    }                                                                                           \
+  } while (0)
+
+namespace {
+
+struct Dev {
+  int fecuda = 0, grid = 0, hydro = 0, timeint = 0;
+  hipStream_t stream = nullptr;
+  int Nx = 0, Ny = 0, Nz = 0, Nh = 0, P = 0, NF = 0, nTKE = 0, nM = 0;
+  long fs = 0, ps = 0;
+  int np = 0;
+  std::vector<float> hprof;
+  float *prof = nullptr;
+  float *S[3] = {nullptr, nullptr, nullptr};
+  float *ls = nullptr, *km = nullptr, *pout = nullptr;
+  float *surf = nullptr;           /* 15 planes of [Ny][P] */
+  float *stg = nullptr;            /* halo-inclusive staging, one 3-D field */
+  long n3h = 0, n2h = 0;
+  float uTop = 0, vTop = 0;
+  float ms_per_step = 0;
+  int gridBlocks[3] = {0, 0, 0};
+  int tb[3] = {0, 0, 0};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};   /* [0]: step without microphysics, [1]: with */
+  int graph_ok = 0;
+} D;
+
+enum { SF_CD = 0, SF_CH, SF_CQ, SF_UST, SF_HF, SF_TSK, SF_IOL, SF_Z0M, SF_Z0T, SF_QF, SF_QSK, SF_T13, SF_T23, SF_TH3, SF_TQ3, SF_N };
+
+inline float *surfp(int i) { return D.surf + (long)i * D.Ny * D.P; }
+
+/* --------------------------------------------------------------------------- */
+/* layout conversion kernels                                                    */
+__global__ void k_host_to_dev(const float *src, float *dst, int Nx, int Ny, int Nz, int Nh, int P) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= Nx) return;
+  const long NYH = Ny + 2 * Nh, NZH = Nz + 2 * Nh;
+  dst[((long)z * Ny + y) * P + x] = src[((long)(x + Nh) * NYH + (y + Nh)) * NZH + (z + Nh)];
+}
+
+/* rule: 0 rho, 1 rho*u, 2 rho*v, 3 rho*w, 4 rho*theta, 5 sgs tke, 6 moisture, 7 pressure */
+__global__ void k_dev_to_host(const float *src, float *dst, int Nx, int Ny, int Nz, int Nh, int P, int rule,
+                              const float *prof, int np, float uTop, float vTop) {
+  const int NZH = Nz + 2 * Nh;
+  int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, i = blockIdx.z;
+  if (k >= NZH) return;
+  const long NYH = Ny + 2 * Nh;
+  int x = i - Nh, y = j - Nh, z = k - Nh;
+  x = ((x % Nx) + Nx) % Nx;
+  y = ((y % Ny) + Ny) % Ny;
+  const long col = (long)y * P + x, ps = (long)Ny * P;
+  float v;
+  if (z >= 0 && z < Nz) v = src[z * ps + col];
+  else if (rule == 0) v = prof[PR_RHO * np + z + 3];
+  else if (rule == 4) v = prof[PR_RTH * np + z + 3];
+  else if (rule == 7) v = (k == 0) ? 0.f : src[(z < 0 ? 0 : Nz - 1) * ps + col];
+  else if (z < 0) v = src[col];
+  else if (rule == 1) v = uTop;
+  else if (rule == 2) v = vTop;
+  else if (rule == 3 || rule == 5) v = 0.f;
+  else v = src[(long)(Nz - 1) * ps + col];
+  dst[((long)i * NYH + j) * NZH + k] = v;
+}
+
+__global__ void k_2d_to_dev(const float *src, float *dst, int Nx, int Ny, int Nh, int P) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= Nx) return;
+  dst[(long)y * P + x] = src[(long)(x + Nh) * (Ny + 2 * Nh) + (y + Nh)];
+}
+__global__ void k_2d_to_host(const float *src, float *dst, int Nx, int Ny, int Nh, int P) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  if (j >= Ny + 2 * Nh) return;
+  int x = (((i - Nh) % Nx) + Nx) % Nx, y = (((j - Nh) % Ny) + Ny) % Ny;
+  dst[(long)i * (Ny + 2 * Nh) + j] = src[(long)y * P + x];
+}
+
+/* --------------------------------------------------------------------------- */
+/* kernel variant table                                                         */
+typedef void (*stage_fn)(StepArgs);
+typedef void (*column_fn)(StepArgs, int, int);
+
+struct Variant {
+  stage_fn plain, phys;
+  column_fn column;
+  const char *name;
+};
+
+template <bool TKE, int NM, int SH, int SE, int SQV, int SQL>
+Variant make_variant(const char *name) {
+  Variant v;
+  v.plain = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>>;
+  v.phys = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
+  v.column = column_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
+  v.name = name;
+  return v;
+}
+
+Variant g_var;
+int g_var_ok = 0;
+
+int select_variant(void) {
+  const int tke = (turbulenceSelector > 0 && TKESelector > 0);
+  const int nm = (moistureSelector > 0) ? moistureNvars : 0;
+  const int sh = advectionSelector, se = tke ? TKEAdvSelector : 3;
+  const int sqv = nm > 0 ? moistureAdvSelectorQv : 3;
+  const int sql = nm > 1 ? (moistureAdvSelectorQi == 0 ? 0 : (moistureAdvSelectorQi == 1 ? 4 : 5)) : 5;
+  g_var_ok = 0;
+  if (tke && nm == 0 && sh == 3 && se == 3) { g_var = make_variant<true, 0, 3, 3, 3, 5>("dry+tke hyb56/hyb56"); g_var_ok = 1; }
+  else if (tke && nm == 2 && sh == 3 && se == 3 && sqv == 3 && sql == 5) { g_var = make_variant<true, 2, 3, 3, 3, 5>("moist2+tke hyb56/hyb56/hyb56/weno5"); g_var_ok = 1; }
+  else if (tke && nm == 2 && sh == 3 && se == 3 && sqv == 3 && sql == 0) { g_var = make_variant<true, 2, 3, 3, 3, 0>("moist2+tke hyb56/.../upstream"); g_var_ok = 1; }
+  else if (tke && nm == 1 && sh == 3 && se == 3 && sqv == 3) { g_var = make_variant<true, 1, 3, 3, 3, 5>("moist1+tke hyb56"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 5 && se == 5) { g_var = make_variant<true, 0, 5, 5, 3, 5>("dry+tke weno5/weno5"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 2 && se == 2) { g_var = make_variant<true, 0, 2, 2, 3, 5>("dry+tke hyb34/hyb34"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 1 && se == 1) { g_var = make_variant<true, 0, 1, 1, 3, 5>("dry+tke quick/quick"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 0 && se == 0) { g_var = make_variant<true, 0, 0, 0, 3, 5>("dry+tke upstream"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 6 && se == 6) { g_var = make_variant<true, 0, 6, 6, 3, 5>("dry+tke centred2"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 4 && se == 4) { g_var = make_variant<true, 0, 4, 4, 3, 5>("dry+tke weno3"); g_var_ok = 1; }
+  else if (tke && nm == 0 && sh == 3 && se == 0) { g_var = make_variant<true, 0, 3, 0, 3, 5>("dry+tke hyb56/upstream"); g_var_ok = 1; }
+  else if (!tke && turbulenceSelector == 0 && nm == 0 && sh == 3) { g_var = make_variant<false, 0, 3, 3, 3, 5>("dry inviscid hyb56"); g_var_ok = 1; }
+  return g_var_ok ? FE_OK : FE_ERR_UNSUPPORTED;
+}
+
+void fill_args(StepArgs &a, int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
+  memset(&a, 0, sizeof a);
+  a.in = in; a.base = base; a.out = out; a.prof = D.prof; a.ls = D.ls; a.km = D.km; a.pout = pout;
+  a.s.cd = surfp(SF_CD); a.s.ch = surfp(SF_CH); a.s.cq = surfp(SF_CQ); a.s.ust = surfp(SF_UST); a.s.hf = surfp(SF_HF);
+  a.s.tsk = surfp(SF_TSK); a.s.iol = surfp(SF_IOL); a.s.z0m = surfp(SF_Z0M); a.s.z0t = surfp(SF_Z0T); a.s.qf = surfp(SF_QF);
+  a.s.qsk = surfp(SF_QSK); a.s.t13 = surfp(SF_T13); a.s.t23 = surfp(SF_T23); a.s.th3 = surfp(SF_TH3); a.s.tq3 = surfp(SF_TQ3);
+  a.Nx = D.Nx; a.Ny = D.Ny; a.Nz = D.Nz; a.P = D.P; a.fs = D.fs; a.ps = D.ps; a.np = D.np;
+  a.dXi = dXi; a.dYi = dYi; a.dZi = dZi;
+  static const float coef[3] = {1.0f / 3.0f, 0.5f, 1.0f};
+  a.coefdt = coef[stage] * dt;
+  a.bh = 1.f - b_hyb; a.be = 1.f - TKEAdvSelector_b_hyb; a.bqv = 1.f - moistureAdvSelectorQv_b;
+  a.g = accel_g; a.fH = corioConstHorz; a.fV = corioConstVert;
+  a.Rv_Rg = Rv_Rg; a.c1p = R_gas / powf(refPressure, R_cp); a.gam = cp_cv;
+  {
+    double c = 1.0, gm = (double)cp_cv;
+    for (int n = 1; n <= 8; ++n) { c = c * (gm - (n - 1)) / n; a.pc[n - 1] = (float)c; }
+  }
+  a.pform = amd_pressureForm;
+  a.filt = filterSelector > 0 ? (float)(0.015625 * filter_6th_coeff / (1.0 * dt)) : 0.f;
+  a.damp = dampingLayerSelector > 0;
+  a.uTop = D.uTop; a.vTop = D.vTop;
+  a.c_k = c_k; a.kappa = kappa; a.refP = refPressure; a.presg = pres_grnd; a.R_cp = R_cp; a.L_v = L_v; a.cp = cp_gas;
+  a.R_vapor = R_vapor; a.condTs = moistureCondTscale; a.condSel = moistureCond; a.condBase = moistureCondBasePres;
+  a.step_it = simTime_it; a.step_itR = simTime_itRestart; a.simTime = simTime; a.dt = dt;
+  a.slSel = surflayerSelector; a.slIdeal = surflayer_idealsine; a.slStab = surflayer_stab;
+  a.slWth = surflayer_wth; a.slWq = surflayer_wq; a.slTr = surflayer_tr; a.slQr = surflayer_qr;
+  a.slTs = surflayer_ideal_ts; a.slTe = surflayer_ideal_te; a.slAmp = surflayer_ideal_amp;
+  a.slQts = surflayer_ideal_qts; a.slQte = surflayer_ideal_qte; a.slQamp = surflayer_ideal_qamp;
+  a.stage = stage;
+  a.emitF = emitF;
+  if (moistureSelector > 0 && moistureCond > 0 && moistureNvars > 1) {
+    int tf = (int)roundf(fmaxf(moistureMPcallTscale, dt) / dt);
+    a.mpActive = (simTime_it % tf) == 0;
+  }
+}
+
+inline int physics_stage(int stage) { return (physics_oneRKonly == 0) || (stage == numRKstages); }
+
+/* launch one RK stage (column pass first when the stage carries the SGS physics) */
+int launch_stage(int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
+  StepArgs a;
+  fill_args(a, stage, in, base, out, emitF, pout);
+  const int phys = physics_stage(stage) && turbulenceSelector > 0;
+  if (physics_stage(stage) && (phys || surflayerSelector > 0)) {
+    dim3 cg((D.Nx + 255) / 256, D.Ny);
+    hipLaunchKernelGGL(g_var.column, cg, dim3(256), 0, D.stream, a, phys, surflayerSelector > 0);
+  }
+  dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY);
+  hipLaunchKernelGGL(phys ? g_var.phys : g_var.plain, grid, dim3(NT), 0, D.stream, a);
+  HIPCHK(hipGetLastError());
+  return FE_OK;
+}
+
+int one_step(float *pout_last) {
+  int rc;
+  if ((rc = launch_stage(0, D.S[0], D.S[0], D.S[1], 0, nullptr))) return rc;
+  if ((rc = launch_stage(1, D.S[1], D.S[0], D.S[2], 0, nullptr))) return rc;
+  if ((rc = launch_stage(2, D.S[2], D.S[0], D.S[0], 0, pout_last))) return rc;
+  return FE_OK;
+}
+
+int upload_field(const float *host, float *dev) {
+  const long bytes = D.n3h * sizeof(float);
+  HIPCHK(hipMemcpyAsync(D.stg, host, bytes, hipMemcpyHostToDevice, D.stream));
+  dim3 g((D.Nx + 63) / 64, D.Ny, D.Nz);
+  hipLaunchKernelGGL(k_host_to_dev, g, dim3(64), 0, D.stream, D.stg, dev, D.Nx, D.Ny, D.Nz, D.Nh, D.P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(D.stream));
+  return FE_OK;
+}
+
+int download_field(const float *dev, float *host, int rule) {
+  const int NZH = D.Nz + 2 * D.Nh;
+  dim3 g((NZH + 63) / 64, D.Ny + 2 * D.Nh, D.Nx + 2 * D.Nh);
+  hipLaunchKernelGGL(k_dev_to_host, g, dim3(64), 0, D.stream, dev, D.stg, D.Nx, D.Ny, D.Nz, D.Nh, D.P, rule, D.prof, D.np, D.uTop, D.vTop);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, D.stg, D.n3h * sizeof(float), hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  return FE_OK;
+}
+
+int upload_2d(const float *host, float *dev) {
+  if (!host) return FE_OK;
+  HIPCHK(hipMemcpyAsync(D.stg, host, D.n2h * sizeof(float), hipMemcpyHostToDevice, D.stream));
+  dim3 g((D.Nx + 63) / 64, D.Ny);
+  hipLaunchKernelGGL(k_2d_to_dev, g, dim3(64), 0, D.stream, D.stg, dev, D.Nx, D.Ny, D.Nh, D.P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(D.stream));
+  return FE_OK;
+}
+
+int download_2d(const float *dev, float *host) {
+  if (!host) return FE_OK;
+  dim3 g((D.Ny + 2 * D.Nh + 63) / 64, D.Nx + 2 * D.Nh);
+  hipLaunchKernelGGL(k_2d_to_host, g, dim3(64), 0, D.stream, dev, D.stg, D.Nx, D.Ny, D.Nh, D.P);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, D.stg, D.n2h * sizeof(float), hipMemcpyDeviceToHost, D.stream));
+  HIPCHK(hipStreamSynchronize(D.stream));
+  return FE_OK;
+}
+
+float *field_ptr(float *S, int f) { return S + (long)f * D.fs; }
+
+int upload_state(void) {
+  int rc;
+  const long n3 = D.n3h;
+  for (int f = 0; f < 5; ++f)
+    if ((rc = upload_field(hydroFlds + f * n3, field_ptr(D.S[0], f)))) return rc;
+  for (int f = 0; f < D.nTKE; ++f)
+    if ((rc = upload_field(sgstkeScalars + f * n3, field_ptr(D.S[0], 5 + f)))) return rc;
+  for (int f = 0; f < D.nM; ++f)
+    if ((rc = upload_field(moistScalars + f * n3, field_ptr(D.S[0], 5 + D.nTKE + f)))) return rc;
+  if (surflayerSelector > 0) {
+    if ((rc = upload_2d(tskin, surfp(SF_TSK)))) return rc;
+    if ((rc = upload_2d(fricVel, surfp(SF_UST)))) return rc;
+    if ((rc = upload_2d(htFlux, surfp(SF_HF)))) return rc;
+    if ((rc = upload_2d(z0m, surfp(SF_Z0M)))) return rc;
+    if ((rc = upload_2d(z0t, surfp(SF_Z0T)))) return rc;
+    if (moistureSelector > 0) {
+      if ((rc = upload_2d(qskin, surfp(SF_QSK)))) return rc;
+      if ((rc = upload_2d(qFlux, surfp(SF_QF)))) return rc;
+    }
+  }
+  return FE_OK;
+}
+
+int download_state(int with_pres) {
+  int rc;
+  const long n3 = D.n3h;
+  for (int f = 0; f < 5; ++f)
+    if ((rc = download_field(field_ptr(D.S[0], f), hydroFlds + f * n3, f))) return rc;
+  for (int f = 0; f < D.nTKE; ++f)
+    if ((rc = download_field(field_ptr(D.S[0], 5 + f), sgstkeScalars + f * n3, 5))) return rc;
+  for (int f = 0; f < D.nM; ++f)
+    if ((rc = download_field(field_ptr(D.S[0], 5 + D.nTKE + f), moistScalars + f * n3, 6))) return rc;
+  if (with_pres && hydroPres)
+    if ((rc = download_field(D.pout, hydroPres, 7))) return rc;
+  if (surflayerSelector > 0) {
+    if ((rc = download_2d(surfp(SF_UST), fricVel))) return rc;
+    if ((rc = download_2d(surfp(SF_HF), htFlux))) return rc;
+    if ((rc = download_2d(surfp(SF_TSK), tskin))) return rc;
+    if ((rc = download_2d(surfp(SF_IOL), invOblen))) return rc;
+    if ((rc = download_2d(surfp(SF_Z0M), z0m))) return rc;
+    if ((rc = download_2d(surfp(SF_Z0T), z0t))) return rc;
+    if ((rc = download_2d(surfp(SF_CD), cdFld))) return rc;
+    if ((rc = download_2d(surfp(SF_CH), chFld))) return rc;
+    if (moistureSelector > 0) {
+      if ((rc = download_2d(surfp(SF_CQ), cqFld))) return rc;
+      if ((rc = download_2d(surfp(SF_QF), qFlux))) return rc;
+      if ((rc = download_2d(surfp(SF_QSK), qskin))) return rc;
+    }
+  }
+  return FE_OK;
+}
+
+long idx3(int i, int j, int k) { return ((long)i * (Nyp + 2 * Nh) + j) * (Nzp + 2 * Nh) + k; }
+
+}  // namespace
+
+/* =========================================================================== */
+extern "C" {
+
+int fecuda_Init(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    fprintf(stderr, "fecuda_Init: no HIP device found\n");
+    return FE_ERR_NO_DEVICE;
+  }
+  if (numProcsX * numProcsY != 1) return FE_ERR_RANKS;
+  HIPCHK(hipSetDevice(mpi_rank_world % n));
+  if (!D.stream) HIPCHK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+  D.tb[0] = tBx; D.tb[1] = tBy; D.tb[2] = tBz;
+  D.fecuda = 1;
+  return FE_OK;
+}
+
+int fecuda_Cleanup(void) {
+  if (D.stream) { hipStreamDestroy(D.stream); D.stream = nullptr; }
+  D.fecuda = 0;
+  return FE_OK;
+}
+
+int fecuda_SetBlocksPerGrid(int Nx_, int Ny_, int Nz_) {
+  /* recorded for reporting; the tile shape of the fused kernels is fixed at 64x4 columns */
+  D.gridBlocks[0] = Nx_; D.gridBlocks[1] = Ny_; D.gridBlocks[2] = Nz_;
+  return FE_OK;
+}
+
+int fecuda_AllocateHaloBuffers(int, int, int, int) { return FE_OK; }
+int fecuda_DeallocateHaloBuffers(void) { return FE_OK; }
+
+/* cuda_gridInit: metric profiles.  Flat terrain only: J31 = J32 = 0 and
+ * J33, D_Jac horizontally uniform (checked over the whole halo-inclusive grid). */
+int cuda_gridInit(void) {
+  if (!D.fecuda) return FE_ERR_STATE;
+  D.Nx = Nxp; D.Ny = Nyp; D.Nz = Nzp; D.Nh = Nh;
+  D.P = (Nxp + 31) / 32 * 32;
+  D.ps = (long)D.Ny * D.P;
+  D.fs = D.ps * D.Nz;
+  D.np = Nzp + 10;
+  D.n3h = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh);
+  D.n2h = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh);
+  const int i0 = iMin, j0 = jMin;
+  for (int i = iMin - Nh; i < iMax + Nh; ++i)
+    for (int j = jMin - Nh; j < jMax + Nh; ++j)
+      for (int k = 0; k < Nzp + 2 * Nh; ++k) {
+        long n = idx3(i, j, k), r = idx3(i0, j0, k);
+        if (J31[n] != 0.f || J32[n] != 0.f || J33[n] != J33[r] || D_Jac[n] != D_Jac[r] || zPos[n] != zPos[r]) {
+          fprintf(stderr, "cuda_gridInit: terrain-following metrics are not supported by this build\n");
+          return FE_ERR_TERRAIN;
+        }
+      }
+  D.hprof.assign((size_t)NPROF * D.np, 0.f);
+  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)row * D.np + z + 3]; };
+  for (int z = -3; z < D.Nz + 7; ++z) {
+    int k = z + Nh;
+    int kc = k < 0 ? 0 : (k >= Nzp + 2 * Nh ? Nzp + 2 * Nh - 1 : k);
+    long r = idx3(i0, j0, kc);
+    P(PR_Z, z) = zPos[r];
+    P(PR_J33, z) = J33[r];
+    P(PR_D, z) = D_Jac[r];
+    P(PR_INVD, z) = invD_Jac[r];
+    P(PR_DELTA, z) = powf(dX * dY * dZ * D_Jac[r], (float)(1.0 / 3.0));
+  }
+  if (D.prof) hipFree(D.prof);
+  HIPCHK(hipMalloc(&D.prof, D.hprof.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(D.prof, D.hprof.data(), D.hprof.size() * sizeof(float), hipMemcpyHostToDevice));
+  D.grid = 1;
+  return FE_OK;
+}
+
+int cuda_gridCleanup(void) {
+  if (D.prof) { hipFree(D.prof); D.prof = nullptr; }
+  D.grid = 0;
+  return FE_OK;
+}
+
+/* cuda_hydroCoreInit: base-state and forcing profiles, device fields. */
+int cuda_hydroCoreInit(void) {
+  if (!D.grid) return FE_ERR_STATE;
+  if (select_variant() != FE_OK) {
+    fprintf(stderr, "cuda_hydroCoreInit: unsupported selector combination (advection %d, TKE adv %d, turbulence %d/%d, moisture %d/%d)\n",
+            advectionSelector, TKEAdvSelector, turbulenceSelector, TKESelector, moistureSelector, moistureNvars);
+    return FE_ERR_UNSUPPORTED;
+  }
+  if (pgfSelector != 1 || buoyancySelector != 1 || hydroBCs != 2) return FE_ERR_UNSUPPORTED;
+  D.nTKE = (turbulenceSelector > 0 && TKESelector > 0) ? 1 : 0;
+  D.nM = moistureSelector > 0 ? moistureNvars : 0;
+  D.NF = 5 + D.nTKE + D.nM;
+  const long fsH = D.n3h;
+  const float *rbs = hydroBaseStateFlds, *tbs = hydroBaseStateFlds + fsH;
+  const int i0 = iMin, j0 = jMin;
+  for (int i = iMin - Nh; i < iMax + Nh; ++i)
+    for (int j = jMin - Nh; j < jMax + Nh; ++j)
+      for (int k = 0; k < Nzp + 2 * Nh; ++k) {
+        long n = idx3(i, j, k), r = idx3(i0, j0, k);
+        if (rbs[n] != rbs[r] || tbs[n] != tbs[r]) {
+          fprintf(stderr, "cuda_hydroCoreInit: base state must be horizontally uniform\n");
+          return FE_ERR_TERRAIN;
+        }
+      }
+  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)row * D.np + z + 3]; };
+  const float c1 = R_gas / powf(refPressure, R_cp);
+  const float pio2 = (float)(0.5 * acos(-1.0));
+  long rtop = idx3(i0, j0, kMax - 1);
+  const float ztop = zPos[rtop];
+  for (int z = -3; z < D.Nz + 7; ++z) {
+    int k = z + Nh;
+    int kc = k < 0 ? 0 : (k >= Nzp + 2 * Nh ? Nzp + 2 * Nh - 1 : k);
+    long r = idx3(i0, j0, kc);
+    P(PR_RHO, z) = rbs[r];
+    P(PR_RTH, z) = tbs[r];
+    P(PR_IRTH, z) = 1.0f / tbs[r];
+    P(PR_PBS, z) = hydroBaseStatePres[r];
+    P(PR_PBSRAW, z) = powf(tbs[r] * c1, cp_cv);
+    float zz = zPos[r];
+    float ubs = (zz < z_Ug) ? U_g * rbs[r] : (U_g + Ug_grad * (zz - z_Ug)) * rbs[r];
+    float vbs = (zz < z_Vg) ? V_g * rbs[r] : (V_g + Vg_grad * (zz - z_Vg)) * rbs[r];
+    P(PR_UG, z) = corioLS_fact * ubs / rbs[r];
+    P(PR_VG, z) = corioLS_fact * vbs / rbs[r];
+    float dmp = 0.f;
+    if (dampingLayerSelector > 0 && zz >= ztop - dampingLayerDepth) {
+      float s = sinf((float)(pio2 * (1.0 - (ztop - zz) / dampingLayerDepth)));
+      dmp = (float)(0.2 * pow((double)s, 2));
+    }
+    P(PR_DAMP, z) = dmp;
+  }
+  {
+    float zt = zPos[rtop], rt = rbs[rtop];
+    D.uTop = (zt < z_Ug) ? U_g * rt : (U_g + Ug_grad * (zt - z_Ug)) * rt;
+    D.vTop = (zt < z_Vg) ? V_g * rt : (V_g + Vg_grad * (zt - z_Vg)) * rt;
+  }
+  HIPCHK(hipMemcpy(D.prof, D.hprof.data(), D.hprof.size() * sizeof(float), hipMemcpyHostToDevice));
+  for (int s = 0; s < 3; ++s) {
+    if (D.S[s]) hipFree(D.S[s]);
+    HIPCHK(hipMalloc(&D.S[s], (size_t)D.NF * D.fs * sizeof(float)));
+    HIPCHK(hipMemset(D.S[s], 0, (size_t)D.NF * D.fs * sizeof(float)));
+  }
+  const size_t nlz = (size_t)(D.Nz + 2) * D.ps;
+  HIPCHK(hipMalloc(&D.ls, nlz * sizeof(float)));
+  HIPCHK(hipMalloc(&D.km, nlz * sizeof(float)));
+  HIPCHK(hipMemset(D.ls, 0, nlz * sizeof(float)));
+  HIPCHK(hipMemset(D.km, 0, nlz * sizeof(float)));
+  HIPCHK(hipMalloc(&D.pout, (size_t)D.fs * sizeof(float)));
+  HIPCHK(hipMemset(D.pout, 0, (size_t)D.fs * sizeof(float)));
+  HIPCHK(hipMalloc(&D.surf, (size_t)SF_N * D.ps * sizeof(float)));
+  HIPCHK(hipMemset(D.surf, 0, (size_t)SF_N * D.ps * sizeof(float)));
+  HIPCHK(hipMalloc(&D.stg, (size_t)D.n3h * sizeof(float)));
+  D.hydro = 1;
+  return FE_OK;
+}
+
+int cuda_hydroCoreCleanup(void) {
+  for (int s = 0; s < 3; ++s) if (D.S[s]) { hipFree(D.S[s]); D.S[s] = nullptr; }
+  float **p[] = {&D.ls, &D.km, &D.pout, &D.surf, &D.stg};
+  for (auto q : p) if (*q) { hipFree(*q); *q = nullptr; }
+  D.hydro = 0;
+  return FE_OK;
+}
+
+int cuda_timeIntInit(void) {
+  if (!D.hydro) return FE_ERR_STATE;
+  if (timeMethod != 0) return FE_ERR_UNSUPPORTED;
+  int rc = upload_state();
+  if (rc) return rc;
+  D.timeint = 1;
+  return FE_OK;
+}
+
+int cuda_timeIntCommence(int it) {
+  if (!D.timeint) return FE_ERR_STATE;
+  int rc;
+  if (it == simTime_itRestart)
+    if ((rc = upload_state())) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, D.stream));
+  for (int s = 0; s < NtBatch; ++s) {
+    if ((rc = one_step(s == NtBatch - 1 ? D.pout : nullptr))) return rc;
+    simTime = simTime + dt;
+    simTime_it = simTime_it + 1;
+  }
+  HIPCHK(hipEventRecord(e1, D.stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  D.ms_per_step = ms / NtBatch;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return download_state(1);
+}
+
+int cuda_timeIntCleanup(void) {
+  D.timeint = 0;
+  return FE_OK;
+}
+
+int fe_upload_state(void) { return D.hydro ? upload_state() : FE_ERR_STATE; }
+int fe_download_state(void) { return D.hydro ? download_state(0) : FE_ERR_STATE; }
+float fe_last_ms_per_step(void) { return D.ms_per_step; }
+const char *fe_kernel_info(void) { return g_var_ok ? g_var.name : "none"; }
+
+int fe_device_steps(int nsteps) {
+  if (!D.timeint) return FE_ERR_STATE;
+  int rc;
+  for (int s = 0; s < nsteps; ++s) {
+    if ((rc = one_step(nullptr))) return rc;
+    simTime = simTime + dt;
+    simTime_it = simTime_it + 1;
+  }
+  HIPCHK(hipStreamSynchronize(D.stream));
+  return FE_OK;
+}
+
+int fe_debug_build_frhs(int stage) {
+  if (!D.hydro) return FE_ERR_STATE;
+  int rc;
+  /* F of the stage evaluated on the current device state S0, into S1 */
+  if ((rc = launch_stage(stage, D.S[0], D.S[0], D.S[1], 1, nullptr))) return rc;
+  HIPCHK(hipStreamSynchronize(D.stream));
+  const long n3 = D.n3h;
+  std::vector<float> tmp((size_t)D.NF * D.fs);
+  HIPCHK(hipMemcpy(tmp.data(), D.S[1], tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+  auto put = [&](float *host, int f) {
+    if (!host) return;
+    memset(host, 0, n3 * sizeof(float));
+    for (int z = 0; z < D.Nz; ++z)
+      for (int y = 0; y < D.Ny; ++y)
+        for (int x = 0; x < D.Nx; ++x)
+          host[idx3(x + Nh, y + Nh, z + Nh)] = tmp[(size_t)f * D.fs + (size_t)z * D.ps + (size_t)y * D.P + x];
+  };
+  for (int f = 0; f < 5; ++f) put(hydroFldsFrhs + f * n3, f);
+  for (int f = 0; f < D.nTKE; ++f) put(sgstkeScalarsFrhs + f * n3, 5 + f);
+  for (int f = 0; f < D.nM; ++f) put(moistScalarsFrhs + f * n3, 5 + D.nTKE + f);
+  /* surface-layer state was advanced by a physics stage; bring it back too */
+  if (physics_stage(stage) && surflayerSelector > 0) {
+    if ((rc = download_2d(surfp(SF_CD), cdFld))) return rc;
+    if ((rc = download_2d(surfp(SF_CH), chFld))) return rc;
+    if ((rc = download_2d(surfp(SF_UST), fricVel))) return rc;
+    if ((rc = download_2d(surfp(SF_IOL), invOblen))) return rc;
+    if ((rc = download_2d(surfp(SF_HF), htFlux))) return rc;
+  }
+  return FE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+/*
+ * fe_kernels.h -- device-side data structures shared by the FastEddy gfx950 kernels.
+ *
+ * Device layout (interior cells only; lateral periodicity and the vertical
+ * boundary conditions are applied while loading):
+ *   state[f][z][y][x], x fastest with row pitch P (multiple of 32 floats),
+ *   fields f = rho, rho*u, rho*v, rho*w, rho*theta, [rho*e], [rho*qv, rho*ql].
+ * Flat terrain (J31 = J32 = 0) makes every metric term a function of z only, so
+ * the metric and the base state travel as 1-D profiles prof[p][z+3],
+ * z in [-3, Nz+3).
+ */
+#ifndef FE_KERNELS_H
+#define FE_KERNELS_H
+#include <hip/hip_runtime.h>
+
+namespace fe {
+
+/* profile rows */
+enum {
+  PR_RHO = 0,   /* rho_BS                                   */
+  PR_RTH,       /* (rho*theta)_BS                           */
+  PR_IRTH,      /* 1/(rho*theta)_BS                         */
+  PR_PBS,       /* (C*(rho*theta)_BS)^(cp/cv)               */
+  PR_Z,         /* zPos                                     */
+  PR_J33, PR_D, PR_INVD,
+  PR_UG,        /* corioLS_fact*u_BS/rho_BS (geostrophic u) */
+  PR_VG,
+  PR_DAMP,      /* 0.2*sin^2(...) inside the damping layer   */
+  PR_DELTA,     /* (dX*dY*dZ*D)^(1/3)                        */
+  PR_PBSRAW,    /* reference-form base pressure term         */
+  NPROF
+};
+
+struct SurfArgs {
+  float *cd, *ch, *cq, *ust, *hf, *tsk, *iol, *z0m, *z0t, *qf, *qsk; /* [Ny][P] */
+  float *t13, *t23, *th3, *tq3;                                    /* surface SGS fluxes [Ny][P] */
+};
+
+struct StepArgs {
+  const float *in;      /* stage input (stencil source)             */
+  const float *base;    /* phi^n                                    */
+  float *out;           /* stage output (or F in emit mode)         */
+  const float *prof;    /* [NPROF][Nz+6]                            */
+  float *ls;            /* SGS length scale  [Nz+2][Ny][P], z' = z+1 */
+  float *km;            /* eddy viscosity    [Nz+2][Ny][P]           */
+  float *pout;          /* optional perturbation-pressure emission [Nz][Ny][P] */
+  SurfArgs s;
+  int Nx, Ny, Nz, P;
+  long fs, ps;          /* field stride, plane stride (Ny*P)        */
+  int np;               /* profile row length Nz+6                  */
+  float dXi, dYi, dZi, coefdt;
+  float bh, be, bqv;    /* (1-b) for the hybrid schemes of each field group */
+  float g, fH, fV;
+  float Rv_Rg, c1p, gam;
+  float pc[8];          /* binomial-series coefficients of (1+r)^gam - 1 */
+  int pform;
+  float filt;           /* 0.015625*coeff/dt, 0 when the filter is off */
+  int damp;
+  float uTop, vTop;     /* momentum in the top halo (geostrophic base state) */
+  float c_k, kappa, refP, presg, R_cp, L_v, cp, R_vapor, condTs;
+  int condSel, condBase, mpActive, mpTheta;
+  int step_it, step_itR;
+  float simTime, dt;
+  int slSel, slIdeal, slStab;
+  float slWth, slWq, slTr, slQr, slTs, slTe, slAmp, slQts, slQte, slQamp;
+  int stage;
+  int emitF;
+};
+
+}  // namespace fe
+#endif

@@ -1,0 +1,157 @@
+/*
+ * fe_driver.c -- single-rank driver: the module call order of
+ * SRC/FEMAIN/FastEddy.c:39-327 (parameters, module GetParams, module Inits,
+ * parameter-error check, grid metrics, base state, then the device-side
+ * cuda_gridInit / fecuda_SetBlocksPerGrid / cuda_hydroCoreInit /
+ * cuda_timeIntInit), plus name-based accessors used by the Python harness.
+ */
+#include <limits.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "fe_host.h"
+#include "../../../include/fasteddy_amd.h"
+
+int tBx, tBy, tBz;
+
+/* SRC/FECUDA/fecuda.c:38-47: the launch block shape stays a mandatory key.  This
+ * build chooses its own tile shapes; tBx/tBy/tBz are validated and recorded. */
+int fecuda_GetParams(void) {
+  int rc = 0, r;
+  r = queryIntegerParameter("tBx", &tBx, 1, INT_MAX, PARAM_MANDATORY); if (r) rc = r;
+  r = queryIntegerParameter("tBy", &tBy, 1, INT_MAX, PARAM_MANDATORY); if (r) rc = r;
+  r = queryIntegerParameter("tBz", &tBz, 1, INT_MAX, PARAM_MANDATORY); if (r) rc = r;
+  return rc;
+}
+
+static int host_ready = 0, device_ready = 0;
+
+int fe_setup_host(const char *paramFile) {
+  if (host_ready) fe_teardown();
+  parameters_init();
+  int rc = parameters_readFile(paramFile);
+  if (rc == PARAM_ERROR_FILE_NOT_FOUND) return rc;
+  int e = 0;
+  if (fempi_GetParams()) e = 1;
+  ioGetParams();
+  fecuda_GetParams();
+  gridGetParams();
+  if (hydro_coreGetParams() < 0) e = 1;
+  timeGetParams();
+  if (getParameterErrors() > 0 || e) {
+    printf("ERROR: %d parameter error(s) in %s\n", getParameterErrors(), paramFile);
+    return getParameterErrors() > 0 ? getParameterErrors() : 1;
+  }
+  if (fempi_Init() || ioInit()) return -1;
+  if (gridInit()) return -2;
+  if (hydro_coreInit()) return -3;
+  if (timeInit()) return -4;
+  if (gridSecondaryPreparations()) return -5;
+  if (hydro_coreSetBaseState()) return -6;
+  if (hydro_corePrepareFromInitialConditions()) return -7;
+  host_ready = 1;
+  return 0;
+}
+
+int fe_setup_device(void) {
+  int rc;
+  if (!host_ready) return -1;
+  if ((rc = fecuda_Init())) return rc;
+  if ((rc = cuda_gridInit())) return rc;
+  if ((rc = fecuda_SetBlocksPerGrid(Nxp + 2 * Nh, Nyp + 2 * Nh, Nzp + 2 * Nh))) return rc;
+  if (numProcsY > 1 && (rc = fecuda_AllocateHaloBuffers(Nxp, Nyp, Nzp, Nh))) return rc;
+  if ((rc = cuda_hydroCoreInit())) return rc;
+  if ((rc = cuda_timeIntInit())) return rc;
+  device_ready = 1;
+  return 0;
+}
+
+int fe_run_batch(void) {
+  if (!device_ready) return -1;
+  return cuda_timeIntCommence(simTime_it);
+}
+
+int fe_teardown(void) {
+  if (device_ready) {
+    cuda_timeIntCleanup();
+    cuda_hydroCoreCleanup();
+    cuda_gridCleanup();
+    fecuda_Cleanup();
+    device_ready = 0;
+  }
+  if (host_ready) {
+    hydro_coreCleanup();
+    gridCleanup();
+    timeCleanup();
+    host_ready = 0;
+  }
+  parameters_clean();
+  return 0;
+}
+
+void *fe_host_array(const char *name, long *count) {
+  const long n3 = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh);
+  const long n2 = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh);
+  struct { const char *nm; float *p; long n; } t[] = {
+    {"hydroFlds", hydroFlds, 5 * n3}, {"hydroFldsFrhs", hydroFldsFrhs, 5 * n3}, {"hydroPres", hydroPres, n3},
+    {"hydroBaseStateFlds", hydroBaseStateFlds, 2 * n3}, {"hydroBaseStatePres", hydroBaseStatePres, n3},
+    {"sgstkeScalars", sgstkeScalars, TKESelector * n3}, {"sgstkeScalarsFrhs", sgstkeScalarsFrhs, TKESelector * n3},
+    {"moistScalars", moistScalars, moistureNvars * n3}, {"moistScalarsFrhs", moistScalarsFrhs, moistureNvars * n3},
+    {"hydroTauFlds", hydroTauFlds, 9 * n3}, {"moistTauFlds", moistTauFlds, 3 * moistureNvars * n3},
+    {"xPos", xPos, n3}, {"yPos", yPos, n3}, {"zPos", zPos, n3}, {"J31", J31, n3}, {"J32", J32, n3}, {"J33", J33, n3},
+    {"D_Jac", D_Jac, n3}, {"invD_Jac", invD_Jac, n3},
+    {"cdFld", cdFld, n2}, {"chFld", chFld, n2}, {"cqFld", cqFld, n2}, {"fricVel", fricVel, n2}, {"htFlux", htFlux, n2},
+    {"tskin", tskin, n2}, {"invOblen", invOblen, n2}, {"z0m", z0m, n2}, {"z0t", z0t, n2}, {"qFlux", qFlux, n2}, {"qskin", qskin, n2},
+  };
+  for (size_t i = 0; i < sizeof t / sizeof t[0]; i++)
+    if (strcmp(t[i].nm, name) == 0) {
+      if (count) *count = t[i].p ? t[i].n : 0;
+      return t[i].p;
+    }
+  if (count) *count = 0;
+  return NULL;
+}
+
+#define INTS(X) X(Nx) X(Ny) X(Nz) X(Nh) X(Nxp) X(Nyp) X(Nzp) X(iMin) X(iMax) X(jMin) X(jMax) X(kMin) X(kMax) \
+  X(hydroBCs) X(advectionSelector) X(TKEAdvSelector) X(pgfSelector) X(buoyancySelector) X(coriolisSelector) \
+  X(turbulenceSelector) X(TKESelector) X(surflayerSelector) X(surflayer_idealsine) X(surflayer_stab) \
+  X(moistureSelector) X(moistureNvars) X(moistureAdvSelectorQv) X(moistureAdvSelectorQi) X(moistureSGSturb) \
+  X(moistureCond) X(moistureCondBasePres) X(filterSelector) X(dampingLayerSelector) X(stabilityScheme) \
+  X(thetaPerturbationSwitch) X(physics_oneRKonly) X(timeMethod) X(Nt) X(NtBatch) X(simTime_it) \
+  X(simTime_itRestart) X(numRKstages) X(tBx) X(tBy) X(tBz) X(numProcsX) X(numProcsY) X(hydroSubGridWrite) \
+  X(hydroForcingWrite) X(hydroForcingLog) X(verticalDeformSwitch) X(coordHorizHalos) X(frqOutput) \
+  X(amd_thetaPerturbationSeed) X(amd_pressureForm) X(diffusionSelector) X(lsfSelector)
+#define FLTS(X) X(d_xi) X(d_eta) X(d_zeta) X(dX) X(dY) X(dZ) X(dXi) X(dYi) X(dZi) X(b_hyb) X(TKEAdvSelector_b_hyb) \
+  X(coriolisLatitude) X(corioConstHorz) X(corioConstVert) X(corioLS_fact) X(c_s) X(c_k) X(surflayer_z0) \
+  X(surflayer_z0t) X(surflayer_tr) X(surflayer_wth) X(surflayer_wq) X(surflayer_qr) X(surflayer_ideal_ts) \
+  X(surflayer_ideal_te) X(surflayer_ideal_amp) X(surflayer_ideal_qts) X(surflayer_ideal_qte) X(surflayer_ideal_qamp) \
+  X(moistureAdvSelectorQv_b) X(moistureCondTscale) X(moistureMPcallTscale) X(filter_6th_coeff) X(dampingLayerDepth) \
+  X(temp_grnd) X(pres_grnd) X(rho_grnd) X(theta_grnd) X(zStableBottom) X(stableGradient) X(zStableBottom2) \
+  X(stableGradient2) X(zStableBottom3) X(stableGradient3) X(U_g) X(V_g) X(z_Ug) X(z_Vg) X(Ug_grad) X(Vg_grad) \
+  X(thetaHeight) X(thetaAmplitude) X(R_gas) X(R_vapor) X(Rv_Rg) X(cp_gas) X(cv_gas) X(accel_g) X(R_cp) X(cp_R) \
+  X(cp_cv) X(refPressure) X(kappa) X(L_v) X(dt) X(simTime) X(verticalDeformFactor) X(verticalDeformQuadCoeff)
+
+int fe_get_int(const char *name, int *value) {
+#define X(v) if (strcmp(name, #v) == 0) { *value = v; return 0; }
+  INTS(X)
+#undef X
+  return -1;
+}
+int fe_set_int(const char *name, int value) {
+#define X(v) if (strcmp(name, #v) == 0) { v = value; return 0; }
+  INTS(X)
+#undef X
+  return -1;
+}
+int fe_get_float(const char *name, float *value) {
+#define X(v) if (strcmp(name, #v) == 0) { *value = v; return 0; }
+  FLTS(X)
+#undef X
+  return -1;
+}
+int fe_set_float(const char *name, float value) {
+#define X(v) if (strcmp(name, #v) == 0) { v = value; return 0; }
+  FLTS(X)
+#undef X
+  return -1;
+}

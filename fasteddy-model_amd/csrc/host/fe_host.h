@@ -1,0 +1,165 @@
+/*
+ * fe_host.h -- host-side module state and entry points of the single-rank
+ * FastEddy time-step build.
+ *
+ * The names, meanings and lifetimes of these process globals follow the
+ * reference's host modules, because the C-ABI device layer
+ * (include/fasteddy_amd.h) consumes them exactly as the reference's CUDA layer
+ * does:
+ *   SRC/GRID/grid.h:24-60, SRC/FEMPI/fempi.h:35-38,
+ *   SRC/HYDRO_CORE/hydro_core.h:35-183,
+ *   SRC/TIME_INTEGRATION/time_integration.h:24-33, SRC/FECUDA/fecuda.h:24-30.
+ * Host arrays are zero-initialised, halo-inclusive and z fastest:
+ *   ijk = i*(Nyp+2Nh)*(Nzp+2Nh) + j*(Nzp+2Nh) + k,   ij = i*(Nyp+2Nh) + j.
+ */
+#ifndef FE_HOST_H
+#define FE_HOST_H
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- PARAMETERS (SRC/PARAMETERS/parameters.h:20-62) ---------------------- */
+#define PARAM_SUCCESS                0
+#define PARAM_WARN_DEFAULTED         1000
+#define PARAM_ERROR_MALLOC           200
+#define PARAM_ERROR_FILE_NOT_FOUND   201
+#define PARAM_ERROR_PATH_NOT_FOUND   202
+#define PARAM_ERROR_INVALID_FLOAT    203
+#define PARAM_ERROR_INVALID_DOUBLE   204
+#define PARAM_ERROR_INVALID_INTEGER  205
+#define PARAM_ERROR_ABSENT           206
+#define PARAM_ERROR_DUPLICATE        207
+#define PARAM_ERROR_OVERWRITE_FAILED 208
+#define PARAM_ERROR_NOT_FOUND        209
+#define PARAM_NOT_REQUIRED 0
+#define PARAM_OPTIONAL     1
+#define PARAM_MANDATORY    2
+
+void parameters_init(void);
+void parameters_clean(void);
+int  parameters_readFile(const char *paramFile);
+int  queryFloatParameter(const char *name, float *var, float min, float max, int reqStatus);
+int  queryDoubleParameter(const char *name, double *var, double min, double max, int reqStatus);
+int  queryIntegerParameter(const char *name, int *var, int min, int max, int reqStatus);
+int  queryStringParameter(const char *name, char **str, int reqStatus);
+int  queryFileParameter(const char *name, char **file, int reqStatus);
+int  queryPathParameter(const char *name, char **path, int reqStatus);
+int  getParameterErrors(void);
+int  countUnusedParameters(void);
+void printUnusedParameters(void);
+void printParameter(const char *name, const char *description);
+void printComment(const char *comment);
+void outputParameters(FILE *out);
+
+/* ---- MEM_UTILS (SRC/MEM_UTILS/mem_utils.c:109-158): aligned, zeroed ------ */
+float *memAllocateFloat2DField(int Nxp, int Nyp, int Nh, const char *name);
+float *memAllocateFloat3DField(int Nxp, int Nyp, int Nzp, int Nh, const char *name);
+float *memAllocateFloat4DField(int Nfld, int Nxp, int Nyp, int Nzp, int Nh, const char *name);
+void   memReleaseFloat(float *p);
+
+/* ---- FEMPI, single rank (SRC/FEMPI/fempi.c:80-154) ---------------------- */
+extern int mpi_rank_world, mpi_size_world, numProcsX, numProcsY, rankXid, rankYid;
+int fempi_GetParams(void);
+int fempi_Init(void);
+
+/* ---- FECUDA host parameters (SRC/FECUDA/fecuda.c:38-47) ------------------ */
+extern int tBx, tBy, tBz;
+
+/* ---- IO parameters (SRC/IO/io.c:65-77) ------------------------------------ */
+extern char *inPath, *inFile, *outPath, *outFileBase;
+extern int frqOutput, ioOutputMode;
+int ioGetParams(void);
+int ioInit(void);
+
+/* ---- GRID (SRC/GRID/grid.h) ----------------------------------------------- */
+extern char *gridFile, *topoFile;
+extern int Nx, Ny, Nz, Nh, Nxp, Nyp, Nzp;
+extern int iMin, iMax, jMin, jMax, kMin, kMax;
+extern float d_xi, d_eta, d_zeta, dX, dY, dZ, dXi, dYi, dZi;
+extern int coordHorizHalos, verticalDeformSwitch;
+extern float verticalDeformFactor, verticalDeformQuadCoeff;
+extern float *xPos, *yPos, *zPos, *topoPos, *J31, *J32, *J33, *D_Jac, *invD_Jac;
+int gridGetParams(void);
+int gridInit(void);
+int gridSecondaryPreparations(void);
+int gridCleanup(void);
+
+/* ---- HYDRO_CORE (SRC/HYDRO_CORE/hydro_core.h) ----------------------------- */
+#define RHO_INDX 0
+#define U_INDX 1
+#define V_INDX 2
+#define W_INDX 3
+#define THETA_INDX 4
+#define RHO_INDX_BS 0
+#define THETA_INDX_BS 1
+extern int Nhydro, hydroBCs, hydroForcingWrite, hydroForcingLog, hydroSubGridWrite;
+extern int pgfSelector, buoyancySelector, coriolisSelector;
+extern float coriolisLatitude, corioConstHorz, corioConstVert, corioLS_fact;
+extern int turbulenceSelector, TKESelector, TKEAdvSelector;
+extern float TKEAdvSelector_b_hyb, c_s, c_k;
+extern int advectionSelector;
+extern float b_hyb;
+extern int diffusionSelector;
+extern float nu_0;
+extern int surflayerSelector, surflayer_idealsine, surflayer_qskin_input, surflayer_stab;
+extern float surflayer_z0, surflayer_z0t, surflayer_tr, surflayer_wth, surflayer_wq, surflayer_qr;
+extern float surflayer_ideal_ts, surflayer_ideal_te, surflayer_ideal_amp;
+extern float surflayer_ideal_qts, surflayer_ideal_qte, surflayer_ideal_qamp;
+extern int lsfSelector;
+extern int moistureSelector, moistureNvars, moistureAdvSelectorQv, moistureAdvSelectorQi;
+extern int moistureSGSturb, moistureCond, moistureCondBasePres;
+extern float moistureAdvSelectorQv_b, moistureCondTscale, moistureMPcallTscale;
+extern int filterSelector;
+extern float filter_6th_coeff;
+extern int dampingLayerSelector;
+extern float dampingLayerDepth;
+extern int stabilityScheme;
+extern float temp_grnd, pres_grnd, rho_grnd, theta_grnd;
+extern float zStableBottom, stableGradient, zStableBottom2, stableGradient2, zStableBottom3, stableGradient3;
+extern float U_g, V_g, z_Ug, z_Vg, Ug_grad, Vg_grad;
+extern int thetaPerturbationSwitch;
+extern float thetaHeight, thetaAmplitude;
+extern int physics_oneRKonly;
+extern float R_gas, R_vapor, Rv_Rg, cp_gas, cv_gas, accel_g, R_cp, cp_R, cp_cv, refPressure, kappa, L_v;
+extern float *hydroFlds, *hydroFldsFrhs, *hydroPres, *hydroBaseStateFlds, *hydroBaseStatePres;
+extern float *sgstkeScalars, *sgstkeScalarsFrhs, *moistScalars, *moistScalarsFrhs;
+extern float *hydroTauFlds, *moistTauFlds;
+extern float *cdFld, *chFld, *cqFld, *fricVel, *htFlux, *tskin, *invOblen, *z0m, *z0t, *qFlux, *qskin;
+/* Build-only keys (namespaced 'amd_', so reference .in files stay valid). */
+extern int amd_thetaPerturbationSeed;   /* srand() seed of the IC perturbation; the reference uses rank+12345 */
+extern int amd_pressureForm;            /* 1 = perturbation form of p' (default), 0 = reference difference of powers */
+int hydro_coreGetParams(void);
+int hydro_coreInit(void);
+int hydro_coreSetBaseState(void);
+int hydro_corePrepareFromInitialConditions(void);
+int hydro_coreCleanup(void);
+
+/* glibc-compatible rand()/srand() sequence (the reference seeds the IC perturbation with
+ * srand(mpi_rank_world+12345), SRC/FEMAIN/FastEddy.c:109). */
+void fe_srand(unsigned int seed);
+int  fe_rand(void);
+
+/* ---- TIME_INTEGRATION (SRC/TIME_INTEGRATION/time_integration.h) ---------- */
+extern int timeMethod, Nt, NtBatch, simTime_it, simTime_itRestart, numRKstages;
+extern float dt, simTime;
+int timeGetParams(void);
+int timeInit(void);
+int timeCleanup(void);
+
+/* ---- Single-rank driver (the call order of SRC/FEMAIN/FastEddy.c:39-485) - */
+int fe_setup_host(const char *paramFile);   /* parse .in, host module inits, grid, base state, ICs */
+int fe_setup_device(void);                  /* fecuda_Init .. cuda_timeIntInit */
+int fe_run_batch(void);                     /* one cuda_timeIntCommence(simTime_it) */
+int fe_teardown(void);
+void *fe_host_array(const char *name, long *count);
+int   fe_get_int(const char *name, int *value);
+int   fe_get_float(const char *name, float *value);
+int   fe_set_float(const char *name, float value);
+int   fe_set_int(const char *name, int value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,356 @@
+/*
+ * fe_hydro_core.c -- host HYDRO_CORE module: parameters, constants, host field
+ * allocation, surface-field initialisation, base state and default initial
+ * conditions.
+ *
+ * Follows SRC/HYDRO_CORE/hydro_core.c: parameters (:197-395), allocations
+ * (:620-770), surface initialisation (:773-861), physical constants (:941-967),
+ * base state schemes 0-2 and 4 (:1053-1250) with the default initial
+ * conditions (geostrophic momentum, rho and rho*theta from the base state,
+ * seeded uniform theta perturbation below thetaHeight).  Large-scale forcing
+ * and molecular diffusion are not part of this build and must stay off.
+ */
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "fe_host.h"
+
+int Nhydro = 5, hydroBCs, hydroForcingWrite, hydroForcingLog, hydroSubGridWrite;
+int pgfSelector, buoyancySelector, coriolisSelector;
+float coriolisLatitude, corioConstHorz, corioConstVert, corioLS_fact;
+int turbulenceSelector, TKESelector, TKEAdvSelector;
+float TKEAdvSelector_b_hyb, c_s, c_k;
+int advectionSelector;
+float b_hyb;
+int diffusionSelector;
+float nu_0;
+int surflayerSelector, surflayer_idealsine, surflayer_qskin_input, surflayer_stab;
+float surflayer_z0, surflayer_z0t, surflayer_tr, surflayer_wth, surflayer_wq, surflayer_qr;
+float surflayer_ideal_ts, surflayer_ideal_te, surflayer_ideal_amp;
+float surflayer_ideal_qts, surflayer_ideal_qte, surflayer_ideal_qamp;
+int lsfSelector;
+int moistureSelector, moistureNvars, moistureAdvSelectorQv, moistureAdvSelectorQi;
+int moistureSGSturb, moistureCond, moistureCondBasePres;
+float moistureAdvSelectorQv_b, moistureCondTscale, moistureMPcallTscale;
+int filterSelector;
+float filter_6th_coeff;
+int dampingLayerSelector;
+float dampingLayerDepth;
+int stabilityScheme;
+float temp_grnd, pres_grnd, rho_grnd, theta_grnd;
+float zStableBottom, stableGradient, zStableBottom2, stableGradient2, zStableBottom3, stableGradient3;
+float U_g, V_g, z_Ug, z_Vg, Ug_grad, Vg_grad;
+int thetaPerturbationSwitch;
+float thetaHeight, thetaAmplitude;
+int physics_oneRKonly;
+float R_gas, R_vapor, Rv_Rg, cp_gas, cv_gas, accel_g, R_cp, cp_R, cp_cv, refPressure, kappa, L_v;
+float *hydroFlds, *hydroFldsFrhs, *hydroPres, *hydroBaseStateFlds, *hydroBaseStatePres;
+float *sgstkeScalars, *sgstkeScalarsFrhs, *moistScalars, *moistScalarsFrhs;
+float *hydroTauFlds, *moistTauFlds;
+float *cdFld, *chFld, *cqFld, *fricVel, *htFlux, *tskin, *invOblen, *z0m, *z0t, *qFlux, *qskin;
+int amd_thetaPerturbationSeed = 12345;
+int amd_pressureForm = 1;
+
+/* -------- glibc TYPE_3 additive-feedback generator (r[i] = r[i-3] + r[i-31]) ----- */
+static unsigned int rstate[31];
+static int rf, rr;
+void fe_srand(unsigned int seed) {
+  int i;
+  int r[31];
+  if (seed == 0) seed = 1;
+  r[0] = (int)seed;
+  for (i = 1; i < 31; i++) {
+    long long t = (16807LL * r[i - 1]) % 2147483647LL;
+    if (t < 0) t += 2147483647LL;
+    r[i] = (int)t;
+  }
+  for (i = 0; i < 31; i++) rstate[i] = (unsigned int)r[i];
+  rf = 3; rr = 0;
+  /* discard the first 310 outputs */
+  for (i = 0; i < 310; i++) {
+    rstate[rf] += rstate[rr];
+    rf = (rf + 1) % 31; rr = (rr + 1) % 31;
+  }
+}
+int fe_rand(void) {
+  unsigned int v = rstate[rf] += rstate[rr];
+  rf = (rf + 1) % 31; rr = (rr + 1) % 31;
+  return (int)(v >> 1);
+}
+
+int hydro_coreGetParams(void) {
+  int rc = 0, r;
+#define CHK(x) do { r = (x); if (r && r != PARAM_WARN_DEFAULTED) rc = r; } while (0)
+  hydroBCs = 0;            CHK(queryIntegerParameter("hydroBCs", &hydroBCs, 2, 2, PARAM_MANDATORY));
+  hydroForcingWrite = 0;   CHK(queryIntegerParameter("hydroForcingWrite", &hydroForcingWrite, 0, 1, PARAM_MANDATORY));
+  hydroForcingLog = 0;     CHK(queryIntegerParameter("hydroForcingLog", &hydroForcingLog, 0, 1, PARAM_MANDATORY));
+  hydroSubGridWrite = 0;   CHK(queryIntegerParameter("hydroSubGridWrite", &hydroSubGridWrite, 0, 1, PARAM_MANDATORY));
+  pgfSelector = 1;         CHK(queryIntegerParameter("pgfSelector", &pgfSelector, 0, 1, PARAM_OPTIONAL));
+  buoyancySelector = 1;    CHK(queryIntegerParameter("buoyancySelector", &buoyancySelector, 0, 1, PARAM_OPTIONAL));
+  coriolisSelector = 0;    CHK(queryIntegerParameter("coriolisSelector", &coriolisSelector, 0, 2, PARAM_MANDATORY));
+  coriolisLatitude = 54.0; CHK(queryFloatParameter("coriolisLatitude", &coriolisLatitude, -90.0, 90.0, PARAM_MANDATORY));
+  turbulenceSelector = 0;  CHK(queryIntegerParameter("turbulenceSelector", &turbulenceSelector, 0, 1, PARAM_MANDATORY));
+  TKESelector = 0;         CHK(queryIntegerParameter("TKESelector", &TKESelector, 0, 1, PARAM_MANDATORY));
+  TKEAdvSelector = 0;      CHK(queryIntegerParameter("TKEAdvSelector", &TKEAdvSelector, 0, 6, PARAM_MANDATORY));
+  TKEAdvSelector_b_hyb = 0.0; CHK(queryFloatParameter("TKEAdvSelector_b_hyb", &TKEAdvSelector_b_hyb, 0.0, 1.0, PARAM_MANDATORY));
+  c_s = 0.18;              CHK(queryFloatParameter("c_s", &c_s, 1e-6, 1e6, PARAM_MANDATORY));
+  c_k = 0.10;              CHK(queryFloatParameter("c_k", &c_k, 1e-6, 1e6, PARAM_MANDATORY));
+  advectionSelector = 0;   CHK(queryIntegerParameter("advectionSelector", &advectionSelector, 0, 6, PARAM_MANDATORY));
+  b_hyb = 0.8;             CHK(queryFloatParameter("b_hyb", &b_hyb, 0.0, 1.0, PARAM_MANDATORY));
+  diffusionSelector = 0;   CHK(queryIntegerParameter("diffusionSelector", &diffusionSelector, 0, 1, PARAM_MANDATORY));
+  nu_0 = 1.0;              CHK(queryFloatParameter("nu_0", &nu_0, 0, FLT_MAX, PARAM_MANDATORY));
+  surflayerSelector = 0;   CHK(queryIntegerParameter("surflayerSelector", &surflayerSelector, 0, 2, PARAM_MANDATORY));
+  surflayer_z0 = 0.1;      CHK(queryFloatParameter("surflayer_z0", &surflayer_z0, 1e-6, 1e+0, PARAM_MANDATORY));
+  surflayer_z0t = 0.1;     CHK(queryFloatParameter("surflayer_z0t", &surflayer_z0t, 1e-6, 1e+1, PARAM_MANDATORY));
+  surflayer_tr = 0.0;      CHK(queryFloatParameter("surflayer_tr", &surflayer_tr, -1e+1, 1e+1, PARAM_MANDATORY));
+  surflayer_wth = 0.0;     CHK(queryFloatParameter("surflayer_wth", &surflayer_wth, -5e+0, 5e+0, PARAM_MANDATORY));
+  surflayer_idealsine = 0; CHK(queryIntegerParameter("surflayer_idealsine", &surflayer_idealsine, 0, 1, PARAM_MANDATORY));
+  surflayer_ideal_ts = 0.0; surflayer_ideal_te = 0.0; surflayer_ideal_amp = 0.1;
+  if (surflayer_idealsine > 0) {
+    CHK(queryFloatParameter("surflayer_ideal_ts", &surflayer_ideal_ts, 0, 1e+5, PARAM_MANDATORY));
+    CHK(queryFloatParameter("surflayer_ideal_te", &surflayer_ideal_te, 0, 1e+5, PARAM_MANDATORY));
+    CHK(queryFloatParameter("surflayer_ideal_amp", &surflayer_ideal_amp, 0, 1e+3, PARAM_MANDATORY));
+  }
+  surflayer_wq = 0.0; surflayer_qr = 0.0; surflayer_qskin_input = 0;
+  surflayer_ideal_qts = 0.0; surflayer_ideal_qte = 0.0; surflayer_ideal_qamp = 0.1;
+  surflayer_stab = 0;      CHK(queryIntegerParameter("surflayer_stab", &surflayer_stab, 0, 1, PARAM_OPTIONAL));
+  lsfSelector = 0;         CHK(queryIntegerParameter("lsfSelector", &lsfSelector, 0, 1, PARAM_MANDATORY));
+  moistureSelector = 0;    CHK(queryIntegerParameter("moistureSelector", &moistureSelector, 0, 1, PARAM_MANDATORY));
+  moistureNvars = 0; moistureAdvSelectorQv = 0; moistureAdvSelectorQv_b = 0.0; moistureAdvSelectorQi = 0;
+  moistureSGSturb = 0; moistureCond = 1; moistureCondTscale = 1.0; moistureCondBasePres = 0; moistureMPcallTscale = 1.0;
+  if (moistureSelector > 0) {
+    CHK(queryIntegerParameter("moistureNvars", &moistureNvars, 0, 2, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("moistureAdvSelectorQv", &moistureAdvSelectorQv, 0, 6, PARAM_MANDATORY));
+    CHK(queryFloatParameter("moistureAdvSelectorQv_b", &moistureAdvSelectorQv_b, 0.0, 1.0, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("moistureAdvSelectorQi", &moistureAdvSelectorQi, 0, 2, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("moistureSGSturb", &moistureSGSturb, 0, 1, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("moistureCond", &moistureCond, 1, 3, PARAM_MANDATORY));
+    CHK(queryFloatParameter("moistureCondTscale", &moistureCondTscale, 1e-4, 1000.0, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("moistureCondBasePres", &moistureCondBasePres, 0, 1, PARAM_MANDATORY));
+    CHK(queryFloatParameter("moistureMPcallTscale", &moistureMPcallTscale, 1e-4, 1000.0, PARAM_MANDATORY));
+    CHK(queryFloatParameter("surflayer_wq", &surflayer_wq, -5e+0, 5e+0, PARAM_MANDATORY));
+    CHK(queryFloatParameter("surflayer_qr", &surflayer_qr, -1e+1, 1e+1, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("surflayer_qskin_input", &surflayer_qskin_input, 0, 1, PARAM_OPTIONAL));
+    if (surflayer_idealsine > 0) {
+      CHK(queryFloatParameter("surflayer_ideal_qts", &surflayer_ideal_qts, 0, 1e+5, PARAM_MANDATORY));
+      CHK(queryFloatParameter("surflayer_ideal_qte", &surflayer_ideal_qte, 0, 1e+5, PARAM_MANDATORY));
+      CHK(queryFloatParameter("surflayer_ideal_qamp", &surflayer_ideal_qamp, 0, 1e+3, PARAM_MANDATORY));
+    }
+  }
+  filterSelector = 0;      CHK(queryIntegerParameter("filterSelector", &filterSelector, 0, 1, PARAM_MANDATORY));
+  filter_6th_coeff = 0.12; CHK(queryFloatParameter("filter_6th_coeff", &filter_6th_coeff, 0.0, 1.0, PARAM_MANDATORY));
+  dampingLayerSelector = 0; CHK(queryIntegerParameter("dampingLayerSelector", &dampingLayerSelector, 0, 1, PARAM_MANDATORY));
+  dampingLayerDepth = 100.0; CHK(queryFloatParameter("dampingLayerDepth", &dampingLayerDepth, 0.0, FLT_MAX, PARAM_MANDATORY));
+  stabilityScheme = 0;     CHK(queryIntegerParameter("stabilityScheme", &stabilityScheme, 0, 4, PARAM_MANDATORY));
+  temp_grnd = 300.0;       CHK(queryFloatParameter("temp_grnd", &temp_grnd, FLT_MIN, FLT_MAX, PARAM_MANDATORY));
+  pres_grnd = 1.0e5;       CHK(queryFloatParameter("pres_grnd", &pres_grnd, FLT_MIN, FLT_MAX, PARAM_MANDATORY));
+  zStableBottom = 1000.0;  CHK(queryFloatParameter("zStableBottom", &zStableBottom, 0, FLT_MAX, PARAM_MANDATORY));
+  stableGradient = 0.1;    CHK(queryFloatParameter("stableGradient", &stableGradient, FLT_MIN, FLT_MAX, PARAM_MANDATORY));
+  zStableBottom2 = 1100.0; CHK(queryFloatParameter("zStableBottom2", &zStableBottom2, 0, FLT_MAX, PARAM_MANDATORY));
+  stableGradient2 = 0.03;  CHK(queryFloatParameter("stableGradient2", &stableGradient2, FLT_MIN, FLT_MAX, PARAM_MANDATORY));
+  zStableBottom3 = 1500.0; CHK(queryFloatParameter("zStableBottom3", &zStableBottom3, 0, FLT_MAX, PARAM_MANDATORY));
+  stableGradient3 = 0.03;  CHK(queryFloatParameter("stableGradient3", &stableGradient3, FLT_MIN, FLT_MAX, PARAM_MANDATORY));
+  U_g = 0.0;               CHK(queryFloatParameter("U_g", &U_g, -FLT_MAX, FLT_MAX, PARAM_MANDATORY));
+  V_g = 0.0;               CHK(queryFloatParameter("V_g", &V_g, -FLT_MAX, FLT_MAX, PARAM_MANDATORY));
+  z_Ug = 10000.0;          CHK(queryFloatParameter("z_Ug", &z_Ug, 0.0, FLT_MAX, PARAM_MANDATORY));
+  z_Vg = 10000.0;          CHK(queryFloatParameter("z_Vg", &z_Vg, 0.0, FLT_MAX, PARAM_MANDATORY));
+  Ug_grad = 0.0;           CHK(queryFloatParameter("Ug_grad", &Ug_grad, -1e2, 1e2, PARAM_MANDATORY));
+  Vg_grad = 0.0;           CHK(queryFloatParameter("Vg_grad", &Vg_grad, -1e2, 1e2, PARAM_MANDATORY));
+  thetaPerturbationSwitch = 0; CHK(queryIntegerParameter("thetaPerturbationSwitch", &thetaPerturbationSwitch, 0, 1, PARAM_MANDATORY));
+  thetaHeight = 0.0;       CHK(queryFloatParameter("thetaHeight", &thetaHeight, 0.0, FLT_MAX, PARAM_MANDATORY));
+  thetaAmplitude = 0.0;    CHK(queryFloatParameter("thetaAmplitude", &thetaAmplitude, 0.0, 2.0, PARAM_MANDATORY));
+  physics_oneRKonly = 1;   CHK(queryIntegerParameter("physics_oneRKonly", &physics_oneRKonly, 0, 1, PARAM_OPTIONAL));
+  amd_thetaPerturbationSeed = 12345;
+  CHK(queryIntegerParameter("amd_thetaPerturbationSeed", &amd_thetaPerturbationSeed, 0, INT_MAX, PARAM_NOT_REQUIRED));
+  amd_pressureForm = 1;
+  CHK(queryIntegerParameter("amd_pressureForm", &amd_pressureForm, 0, 1, PARAM_NOT_REQUIRED));
+#undef CHK
+  if (diffusionSelector != 0 || lsfSelector != 0) {
+    printf("ERROR: diffusionSelector and lsfSelector must be 0 in this build\n");
+    return -1;
+  }
+  return rc;
+}
+
+int hydro_coreInit(void) {
+  const int NxH = Nxp + 2 * Nh, NyH = Nyp + 2 * Nh;
+  hydroFlds = memAllocateFloat4DField(Nhydro, Nxp, Nyp, Nzp, Nh, "hydroFlds");
+  hydroPres = memAllocateFloat3DField(Nxp, Nyp, Nzp, Nh, "hydroPres");
+  hydroFldsFrhs = memAllocateFloat4DField(Nhydro, Nxp, Nyp, Nzp, Nh, "hydroFldsFrhs");
+  if (!hydroFlds || !hydroPres || !hydroFldsFrhs) return -1;
+  if (TKESelector > 0) {
+    sgstkeScalars = memAllocateFloat4DField(TKESelector, Nxp, Nyp, Nzp, Nh, "sgstkeScalars");
+    sgstkeScalarsFrhs = memAllocateFloat4DField(TKESelector, Nxp, Nyp, Nzp, Nh, "sgstkeScalarsFrhs");
+    if (!sgstkeScalars || !sgstkeScalarsFrhs) return -1;
+  }
+  hydroBaseStateFlds = memAllocateFloat4DField(2, Nxp, Nyp, Nzp, Nh, "hydroBaseStateFlds");
+  hydroBaseStatePres = memAllocateFloat3DField(Nxp, Nyp, Nzp, Nh, "hydroBaseStatePres");
+  if (!hydroBaseStateFlds || !hydroBaseStatePres) return -1;
+  if (turbulenceSelector > 0) {
+    hydroTauFlds = memAllocateFloat4DField(9, Nxp, Nyp, Nzp, Nh, "hydroTauFlds");
+    if (!hydroTauFlds) return -1;
+  }
+  kappa = 0.4;
+  if (surflayerSelector > 0) {
+    cdFld = memAllocateFloat2DField(Nxp, Nyp, Nh, "cdFld");
+    chFld = memAllocateFloat2DField(Nxp, Nyp, Nh, "chFld");
+    cqFld = memAllocateFloat2DField(Nxp, Nyp, Nh, "cqFld");
+    fricVel = memAllocateFloat2DField(Nxp, Nyp, Nh, "fricVel");
+    htFlux = memAllocateFloat2DField(Nxp, Nyp, Nh, "htFlux");
+    tskin = memAllocateFloat2DField(Nxp, Nyp, Nh, "tskin");
+    invOblen = memAllocateFloat2DField(Nxp, Nyp, Nh, "invOblen");
+    qFlux = memAllocateFloat2DField(Nxp, Nyp, Nh, "qFlux");
+    qskin = memAllocateFloat2DField(Nxp, Nyp, Nh, "qskin");
+    z0m = memAllocateFloat2DField(Nxp, Nyp, Nh, "z0m");
+    z0t = memAllocateFloat2DField(Nxp, Nyp, Nh, "z0t");
+    /* Surface initialisation (hydro_core.c:835-860).  It runs before the
+     * physical constants are assigned (:941), so refPressure and R_cp are still
+     * zero there and the skin temperature is temp_grnd/pow(0,0) = temp_grnd.
+     * cd/ch/cq are recomputed on the device at the first step. */
+    for (int i = 0; i < NxH; i++)
+      for (int j = 0; j < NyH; j++) {
+        long ij = (long)i * NyH + j;
+        z0m[ij] = surflayer_z0;
+        z0t[ij] = surflayer_z0t;
+        if (surflayerSelector == 1) {
+          htFlux[ij] = surflayer_wth;
+          if (moistureSelector > 0) qFlux[ij] = surflayer_wq;
+        }
+        tskin[ij] = temp_grnd / pow((0.0f / pres_grnd), 0.0f);
+        if (moistureSelector > 0) qskin[ij] = 0.0f;
+      }
+  }
+  if (moistureSelector > 0) {
+    moistScalars = memAllocateFloat4DField(moistureNvars, Nxp, Nyp, Nzp, Nh, "moistScalars");
+    moistScalarsFrhs = memAllocateFloat4DField(moistureNvars, Nxp, Nyp, Nzp, Nh, "moistScalarsFrhs");
+    if (!moistScalars || !moistScalarsFrhs) return -1;
+    if (moistureSGSturb > 0) {
+      moistTauFlds = memAllocateFloat4DField(moistureNvars * 3, Nxp, Nyp, Nzp, Nh, "moistTauFlds");
+      if (!moistTauFlds) return -1;
+    }
+  }
+  /* physical constants (hydro_core.c:941-967) */
+  accel_g = 9.81;
+  R_gas = 287.04;
+  R_vapor = 461.60;
+  cv_gas = 718.0;
+  cp_gas = R_gas + cv_gas;
+  L_v = 2.5e6;
+  R_cp = R_gas / cp_gas;
+  cp_R = cp_gas / R_gas;
+  cp_cv = cp_gas / cv_gas;
+  refPressure = 1.0e5;
+  Rv_Rg = R_vapor / R_gas;
+  float pi = acos(-1);
+  if (coriolisSelector > 0) {
+    corioConstHorz = 1.45842e-4 * sin(pi / 180.0 * coriolisLatitude);
+    corioConstVert = (coriolisSelector > 1) ? 1.45842e-4 * cos(pi / 180.0 * coriolisLatitude) : 0.0;
+    corioLS_fact = 1.0;
+  } else {
+    corioConstHorz = 0.0;
+    corioConstVert = 0.0;
+  }
+  return 0;
+}
+
+#define IJK(i, j, k) ((long)(i) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh) + (long)(j) * (Nzp + 2 * Nh) + (k))
+
+/* Base state and default initial conditions, hydro_core.c:1053-1250. */
+int hydro_coreSetBaseState(void) {
+  const long fs = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh);
+  float constant_1 = R_gas / (pow(refPressure, R_cp));
+  float *rhoBase = &hydroBaseStateFlds[RHO_INDX_BS * fs];
+  float *thetaBase = &hydroBaseStateFlds[THETA_INDX_BS * fs];
+  rho_grnd = pres_grnd / (R_gas * temp_grnd);
+  theta_grnd = temp_grnd * pow(pres_grnd / refPressure, -R_cp);
+  if (stabilityScheme == 3) {
+    printf("ERROR: stabilityScheme 3 takes the base state from an input file, which this build does not read\n");
+    return -1;
+  }
+  for (int i = iMin - Nh; i < iMax + Nh; i++)
+    for (int j = jMin - Nh; j < jMax + Nh; j++)
+      for (int k = kMin - Nh; k < kMax + Nh; k++) {
+        long n = IJK(i, j, k);
+        float z = zPos[n];
+        if (stabilityScheme == 0) {
+          rhoBase[n] = rho_grnd;
+          thetaBase[n] = rho_grnd * theta_grnd;
+          hydroBaseStatePres[n] = pow(thetaBase[n] * constant_1, cp_cv);
+          continue;
+        }
+        if (stabilityScheme == 4) {
+          rhoBase[n] = 1.1f;
+          thetaBase[n] = 1.1f * theta_grnd;
+          hydroBaseStatePres[n] = pow(thetaBase[n] * constant_1, cp_cv);
+          continue;
+        }
+        const double pg = pow(pres_grnd / refPressure, R_cp);
+        if (z <= zStableBottom) {
+          thetaBase[n] = theta_grnd;
+          hydroBaseStatePres[n] = refPressure * pow((-accel_g / cp_gas) * (z / theta_grnd) + pg, cp_R);
+        } else if (stabilityScheme == 1 || z <= zStableBottom2) {
+          thetaBase[n] = theta_grnd + stableGradient * (z - zStableBottom);
+          hydroBaseStatePres[n] = refPressure * pow((-accel_g / cp_gas) * (zStableBottom / theta_grnd
+                                   + (1.0 / stableGradient) * log(1.0 + stableGradient * (z - zStableBottom) / theta_grnd)) + pg, cp_R);
+        } else if (z <= zStableBottom3) {
+          thetaBase[n] = theta_grnd + stableGradient * (zStableBottom2 - zStableBottom) + stableGradient2 * (z - zStableBottom2);
+          hydroBaseStatePres[n] = refPressure * pow((-accel_g / cp_gas) * (zStableBottom / theta_grnd
+                                   + (1.0 / stableGradient) * log(1.0 + stableGradient * (zStableBottom2 - zStableBottom) / theta_grnd)
+                                   + (1.0 / stableGradient2) * log(1.0 + stableGradient2 * (z - zStableBottom2) / (theta_grnd + stableGradient * (zStableBottom2 - zStableBottom))))
+                                   + pg, cp_R);
+        } else {
+          thetaBase[n] = theta_grnd + stableGradient * (zStableBottom2 - zStableBottom) + stableGradient2 * (zStableBottom3 - zStableBottom2)
+                       + stableGradient3 * (z - zStableBottom3);
+          hydroBaseStatePres[n] = refPressure * pow((-accel_g / cp_gas) * (zStableBottom / theta_grnd
+                                   + (1.0 / stableGradient) * log(1.0 + stableGradient * (zStableBottom2 - zStableBottom) / theta_grnd)
+                                   + (1.0 / stableGradient2) * log(1.0 + stableGradient2 * (zStableBottom3 - zStableBottom2) / (theta_grnd + stableGradient * (zStableBottom2 - zStableBottom)))
+                                   + (1.0 / stableGradient3) * log(1.0 + stableGradient3 * (z - zStableBottom3) / (theta_grnd + stableGradient * (zStableBottom2 - zStableBottom)
+                                                                    + stableGradient2 * (zStableBottom3 - zStableBottom2))))
+                                   + pg, cp_R);
+        }
+        float BS_Temp = thetaBase[n] * pow(hydroBaseStatePres[n] / refPressure, R_cp);
+        rhoBase[n] = hydroBaseStatePres[n] / (BS_Temp * R_gas);
+        thetaBase[n] = thetaBase[n] * rhoBase[n];
+        hydroBaseStatePres[n] = pow(thetaBase[n] * constant_1, cp_cv);
+      }
+  /* default initial conditions (no input file): geostrophic momentum, w = 0 */
+  float *uIni = &hydroFlds[U_INDX * fs], *vIni = &hydroFlds[V_INDX * fs], *wIni = &hydroFlds[W_INDX * fs];
+  for (int i = iMin - Nh; i < iMax + Nh; i++)
+    for (int j = jMin - Nh; j < jMax + Nh; j++)
+      for (int k = kMin; k < kMax + Nh; k++) {
+        long n = IJK(i, j, k);
+        uIni[n] = (zPos[n] < z_Ug) ? U_g * rhoBase[n] : (U_g + Ug_grad * (zPos[n] - z_Ug)) * rhoBase[n];
+        vIni[n] = (zPos[n] < z_Vg) ? V_g * rhoBase[n] : (V_g + Vg_grad * (zPos[n] - z_Vg)) * rhoBase[n];
+        wIni[n] = 0.0f;
+      }
+  memcpy(&hydroFlds[RHO_INDX * fs], rhoBase, sizeof(float) * fs);
+  memcpy(&hydroFlds[THETA_INDX * fs], thetaBase, sizeof(float) * fs);
+  if (thetaPerturbationSwitch == 1) {
+    float *rb = &hydroFlds[RHO_INDX * fs], *fb = &hydroFlds[THETA_INDX * fs];
+    fe_srand((unsigned int)amd_thetaPerturbationSeed);
+    for (int i = iMin; i < iMax; i++)
+      for (int j = jMin; j < jMax; j++)
+        for (int k = kMin; k < kMax; k++) {
+          long n = IJK(i, j, k);
+          if (zPos[n] <= thetaHeight)
+            fb[n] = (fb[n] / rb[n] + 2.0 * thetaAmplitude * (((float)fe_rand() / (float)(2147483647)) - 0.5)) * rb[n];
+        }
+  }
+  return 0;
+}
+
+int hydro_corePrepareFromInitialConditions(void) { return 0; }
+
+int hydro_coreCleanup(void) {
+  float **arrs[] = {&hydroFlds, &hydroFldsFrhs, &hydroPres, &hydroBaseStateFlds, &hydroBaseStatePres,
+                    &sgstkeScalars, &sgstkeScalarsFrhs, &moistScalars, &moistScalarsFrhs, &hydroTauFlds,
+                    &moistTauFlds, &cdFld, &chFld, &cqFld, &fricVel, &htFlux, &tskin, &invOblen, &z0m, &z0t, &qFlux, &qskin};
+  for (size_t a = 0; a < sizeof(arrs) / sizeof(arrs[0]); a++) {
+    memReleaseFloat(*arrs[a]);
+    *arrs[a] = NULL;
+  }
+  return 0;
+}

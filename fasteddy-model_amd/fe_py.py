@@ -1,0 +1,392 @@
+"""Python harness over the FastEddy MI355X C-ABI library (ctypes, numpy views).
+
+The product is libfasteddy_amd.so: the host C modules (parameters, grid,
+hydro_core, time integration) and the gfx950 device layer behind the
+reference's entry points (fecuda_*, cuda_gridInit, cuda_hydroCoreInit,
+cuda_timeIntInit/Commence/Cleanup).  This module only writes .in files, calls
+those entry points and exposes the host arrays as numpy views in the reference
+layout (halo-inclusive, z fastest).
+
+Oracle helpers at the bottom wrap the CPU oracle (oracle/liboracle_f32.so,
+oracle/liboracle_f64.so); they are test infrastructure and never used by the
+product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libfasteddy_amd.so")
+
+# ----------------------------------------------------------------------------
+# reference-style parameter sets
+
+# Keys common to the dry CBL cases (reference tutorials/examples/Example02_CBL.in
+# with the config's values).
+CBL_BASE = {
+    "numProcsX": 1, "numProcsY": 1,
+    "tBx": 1, "tBy": 4, "tBz": 64,
+    "outPath": "./", "outFileBase": "FE_CBL", "frqOutput": 1000000, "ioOutputMode": 1,
+    "Nh": 3, "coordHorizHalos": 1, "verticalDeformSwitch": 0,
+    "verticalDeformFactor": 0.8, "verticalDeformQuadCoeff": 0.0,
+    "timeMethod": 0,
+    "hydroBCs": 2, "hydroForcingWrite": 0, "hydroSubGridWrite": 0, "hydroForcingLog": 0,
+    "advectionSelector": 3, "b_hyb": 0.0,
+    "moistureSelector": 0,
+    "coriolisSelector": 1, "coriolisLatitude": 43.2885,
+    "turbulenceSelector": 1, "TKESelector": 1, "TKEAdvSelector": 3, "TKEAdvSelector_b_hyb": 0.0,
+    "c_s": 0.18, "c_k": 0.10,
+    "diffusionSelector": 0, "nu_0": 0.0,
+    "filterSelector": 0, "filter_6th_coeff": 0.03,
+    "dampingLayerSelector": 1, "dampingLayerDepth": 200.0,
+    "surflayerSelector": 1, "surflayer_z0": 0.05, "surflayer_z0t": 0.05, "surflayer_wth": 0.35,
+    "surflayer_tr": 0.0, "surflayer_idealsine": 0,
+    "stabilityScheme": 1, "temp_grnd": 300.0, "pres_grnd": 100000.0,
+    "zStableBottom": 600.0, "stableGradient": 0.008,
+    "zStableBottom2": 6000.0, "stableGradient2": 0.08, "zStableBottom3": 50000.0, "stableGradient3": 0.003,
+    "thetaPerturbationSwitch": 1, "thetaHeight": 400.0, "thetaAmplitude": 0.25,
+    "U_g": 10.0, "V_g": 0.0, "z_Ug": 10000.0, "z_Vg": 10000.0, "Ug_grad": 0.0, "Vg_grad": 0.0,
+    "lsfSelector": 0,
+}
+
+# Dry convective boundary layer, 64x64x48 at 20 m, dt 0.04 s, 200 steps.
+CFG0 = dict(CBL_BASE, Nx=64, Ny=64, Nz=48, d_xi=20.0, d_eta=20.0, d_zeta=20.0, dt=0.04, Nt=200, NtBatch=200)
+# Dry CBL at production size, 512x512x128 at 10 m, dt 0.02 s, 1000 steps.
+CFG1 = dict(CBL_BASE, Nx=512, Ny=512, Nz=128, d_xi=10.0, d_eta=10.0, d_zeta=10.0, dt=0.02, Nt=1000, NtBatch=1000)
+# Moist shallow cumulus, 256x256x160 at 25 m, dt 0.05 s, 600 steps.
+CFG2 = dict(CBL_BASE, Nx=256, Ny=256, Nz=160, d_xi=25.0, d_eta=25.0, d_zeta=25.0, dt=0.05, Nt=600, NtBatch=600,
+            stabilityScheme=2, temp_grnd=298.7, zStableBottom=520.0, stableGradient=0.00385,
+            zStableBottom2=1500.0, stableGradient2=0.011, zStableBottom3=100000.0, stableGradient3=0.011,
+            thetaHeight=1600.0, thetaAmplitude=0.1, U_g=-10.0,
+            moistureSelector=1, moistureNvars=2, moistureAdvSelectorQv=3, moistureAdvSelectorQv_b=0.0,
+            moistureAdvSelectorQi=2, moistureSGSturb=1, moistureCond=3, moistureCondTscale=1.0,
+            moistureCondBasePres=1, moistureMPcallTscale=1.0, surflayer_wth=0.008, surflayer_wq=0.052,
+            surflayer_qr=0.0)
+
+
+def write_in(params, path):
+    """Write a reference-grammar .in file (key = value # comment)."""
+    with open(path, "w") as fh:
+        fh.write("Description = generated case\n")
+        for k, v in params.items():
+            fh.write(f"{k} = {v}\n")
+    return path
+
+
+class Model:
+    """One model instance over the product library (process-global state: one at a time)."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+            L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+            L.fe_setup_host.argtypes = [C.c_char_p]
+            L.fe_host_array.restype = C.c_void_p
+            L.fe_host_array.argtypes = [C.c_char_p, C.POINTER(C.c_long)]
+            L.fe_get_int.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+            L.fe_get_float.argtypes = [C.c_char_p, C.POINTER(C.c_float)]
+            L.fe_set_int.argtypes = [C.c_char_p, C.c_int]
+            L.fe_set_float.argtypes = [C.c_char_p, C.c_float]
+            L.fe_last_ms_per_step.restype = C.c_float
+            L.fe_kernel_info.restype = C.c_char_p
+            L.cuda_timeIntCommence.argtypes = [C.c_int]
+            L.fe_debug_build_frhs.argtypes = [C.c_int]
+            L.fe_device_steps.argtypes = [C.c_int]
+            L.fe_rand.restype = C.c_int
+            L.fe_srand.argtypes = [C.c_uint]
+            L.getParameterErrors.restype = C.c_int
+            L.countUnusedParameters.restype = C.c_int
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, params, workdir, device=True):
+        self.L = Model.lib()
+        self.params = dict(params)
+        os.makedirs(workdir, exist_ok=True)
+        self.inpath = write_in(self.params, os.path.join(workdir, "case.in"))
+        rc = self.L.fe_setup_host(self.inpath.encode())
+        if rc != 0:
+            raise RuntimeError(f"fe_setup_host failed with {rc}")
+        self.device = False
+        if device:
+            rc = self.L.fe_setup_device()
+            if rc != 0:
+                self.L.fe_teardown()
+                raise RuntimeError(f"fe_setup_device failed with {rc}")
+            self.device = True
+
+    # -- accessors --
+    def geti(self, name):
+        v = C.c_int()
+        if self.L.fe_get_int(name.encode(), C.byref(v)) != 0:
+            raise KeyError(name)
+        return v.value
+
+    def getf(self, name):
+        v = C.c_float()
+        if self.L.fe_get_float(name.encode(), C.byref(v)) != 0:
+            raise KeyError(name)
+        return v.value
+
+    def seti(self, name, value):
+        self.L.fe_set_int(name.encode(), int(value))
+
+    def setf(self, name, value):
+        self.L.fe_set_float(name.encode(), float(value))
+
+    @property
+    def shape3(self):
+        nh = self.geti("Nh")
+        return (self.geti("Nx") + 2 * nh, self.geti("Ny") + 2 * nh, self.geti("Nz") + 2 * nh)
+
+    @property
+    def shape2(self):
+        s = self.shape3
+        return (s[0], s[1])
+
+    def array(self, name):
+        n = C.c_long()
+        p = self.L.fe_host_array(name.encode(), C.byref(n))
+        if not p or n.value == 0:
+            return None
+        flat = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n.value,))
+        s3 = self.shape3
+        n3 = s3[0] * s3[1] * s3[2]
+        n2 = s3[0] * s3[1]
+        if n.value % n3 == 0 and n.value >= n3:
+            return flat.reshape((n.value // n3,) + s3)
+        return flat.reshape(self.shape2)
+
+    def interior(self, a):
+        nh = self.geti("Nh")
+        if a.ndim == 4:
+            return a[:, nh:-nh, nh:-nh, nh:-nh]
+        return a[nh:-nh, nh:-nh, nh:-nh]
+
+    # -- device --
+    def build_frhs(self, stage):
+        rc = self.L.fe_debug_build_frhs(int(stage))
+        if rc != 0:
+            raise RuntimeError(f"fe_debug_build_frhs failed with {rc}")
+
+    def upload(self):
+        rc = self.L.fe_upload_state()
+        if rc != 0:
+            raise RuntimeError(f"fe_upload_state failed with {rc}")
+
+    def download(self):
+        rc = self.L.fe_download_state()
+        if rc != 0:
+            raise RuntimeError(f"fe_download_state failed with {rc}")
+
+    def commence(self):
+        rc = self.L.cuda_timeIntCommence(self.geti("simTime_it"))
+        if rc != 0:
+            raise RuntimeError(f"cuda_timeIntCommence failed with {rc}")
+
+    def device_steps(self, n):
+        rc = self.L.fe_device_steps(int(n))
+        if rc != 0:
+            raise RuntimeError(f"fe_device_steps failed with {rc}")
+
+    def close(self):
+        self.L.fe_teardown()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# ----------------------------------------------------------------------------
+# Oracle (test infrastructure only)
+
+class FeCfg(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("Nh", C.c_int),
+        ("d_xi", C.c_double), ("d_eta", C.c_double), ("d_zeta", C.c_double),
+        ("hydroBCs", C.c_int),
+        ("advectionSelector", C.c_int), ("b_hyb", C.c_double),
+        ("TKEAdvSelector", C.c_int), ("TKEAdvSelector_b_hyb", C.c_double),
+        ("pgfSelector", C.c_int), ("buoyancySelector", C.c_int), ("coriolisSelector", C.c_int),
+        ("coriolisLatitude", C.c_double),
+        ("turbulenceSelector", C.c_int), ("TKESelector", C.c_int),
+        ("c_s", C.c_double), ("c_k", C.c_double),
+        ("surflayerSelector", C.c_int),
+        ("surflayer_z0", C.c_double), ("surflayer_z0t", C.c_double), ("surflayer_wth", C.c_double),
+        ("surflayer_tr", C.c_double), ("surflayer_wq", C.c_double), ("surflayer_qr", C.c_double),
+        ("surflayer_idealsine", C.c_int),
+        ("surflayer_ideal_ts", C.c_double), ("surflayer_ideal_te", C.c_double), ("surflayer_ideal_amp", C.c_double),
+        ("surflayer_ideal_qts", C.c_double), ("surflayer_ideal_qte", C.c_double), ("surflayer_ideal_qamp", C.c_double),
+        ("surflayer_stab", C.c_int),
+        ("moistureSelector", C.c_int), ("moistureNvars", C.c_int), ("moistureAdvSelectorQv", C.c_int),
+        ("moistureAdvSelectorQv_b", C.c_double),
+        ("moistureAdvSelectorQi", C.c_int), ("moistureSGSturb", C.c_int), ("moistureCond", C.c_int),
+        ("moistureCondTscale", C.c_double),
+        ("moistureCondBasePres", C.c_int),
+        ("moistureMPcallTscale", C.c_double),
+        ("filterSelector", C.c_int), ("filter_6th_coeff", C.c_double),
+        ("dampingLayerSelector", C.c_int), ("dampingLayerDepth", C.c_double),
+        ("temp_grnd", C.c_double), ("pres_grnd", C.c_double),
+        ("U_g", C.c_double), ("V_g", C.c_double), ("z_Ug", C.c_double), ("z_Vg", C.c_double),
+        ("Ug_grad", C.c_double), ("Vg_grad", C.c_double),
+        ("physics_oneRKonly", C.c_int),
+        ("dt", C.c_double),
+    ]
+
+
+OR_IDS = ["HYDRO", "HYDRO_F", "TKE", "TKE_F", "MOIST", "MOIST_F", "BS", "ZPOS", "J31", "J32", "J33", "DJAC",
+          "INVDJAC", "PRES", "RHOINV", "FACEVEL", "TAU", "KM", "LS", "DEDXI", "MOISTTAU", "FCOND", "TIME0",
+          "CD", "CH", "CQ", "FRICVEL", "HTFLUX", "TSKIN", "INVOBLEN", "Z0M", "Z0T", "QFLUX", "QSKIN"]
+OR = {n: i for i, n in enumerate(OR_IDS)}
+
+_DEFAULTS = {"physics_oneRKonly": 1, "pgfSelector": 1, "buoyancySelector": 1, "surflayer_stab": 0,
+             "moistureNvars": 0, "moistureAdvSelectorQv": 0, "moistureAdvSelectorQv_b": 0.0,
+             "moistureAdvSelectorQi": 0, "moistureSGSturb": 0, "moistureCond": 1, "moistureCondTscale": 1.0,
+             "moistureCondBasePres": 0, "moistureMPcallTscale": 1.0, "surflayer_wq": 0.0, "surflayer_qr": 0.0,
+             "surflayer_ideal_ts": 0.0, "surflayer_ideal_te": 0.0, "surflayer_ideal_amp": 0.1,
+             "surflayer_ideal_qts": 0.0, "surflayer_ideal_qte": 0.0, "surflayer_ideal_qamp": 0.1}
+
+
+def make_cfg(params):
+    """fe_cfg from a parameter dict; float parameters are rounded to float32 as sscanf('%f') does."""
+    cfg = FeCfg()
+    for name, ctype in FeCfg._fields_:
+        v = params.get(name, _DEFAULTS.get(name, 0))
+        if ctype is C.c_double:
+            setattr(cfg, name, float(np.float32(v)))
+        else:
+            setattr(cfg, name, int(v))
+    return cfg
+
+
+class Oracle:
+    """CPU oracle handle (REAL=float for bits=32, REAL=double for bits=64)."""
+
+    _libs = {}
+
+    def __init__(self, params, bits=32, threads=1):
+        if bits not in Oracle._libs:
+            p = os.path.join(REPO, "oracle", f"liboracle_f{bits}.so")
+            L = C.CDLL(p)
+            L.oracle_create.restype = C.c_void_p
+            L.oracle_create.argtypes = [C.POINTER(FeCfg)]
+            L.oracle_destroy.argtypes = [C.c_void_p]
+            L.oracle_array.restype = C.c_void_p
+            L.oracle_array.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
+            L.oracle_build_frhs.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_int]
+            L.oracle_rk_stage.argtypes = [C.c_void_p, C.c_int]
+            L.oracle_steps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_int]
+            L.oracle_apply_bcs.argtypes = [C.c_void_p]
+            L.oracle_set_threads.argtypes = [C.c_int]
+            Oracle._libs[bits] = L
+        self.L = Oracle._libs[bits]
+        self.L.oracle_set_threads(int(threads))
+        self.bits = bits
+        self.dtype = np.float32 if bits == 32 else np.float64
+        self.params = dict(params)
+        self.cfg = make_cfg(params)
+        self.h = self.L.oracle_create(C.byref(self.cfg))
+        if not self.h:
+            raise MemoryError("oracle_create failed")
+        nh = int(params["Nh"])
+        self.shape3 = (int(params["Nx"]) + 2 * nh, int(params["Ny"]) + 2 * nh, int(params["Nz"]) + 2 * nh)
+        self.nh = nh
+        self.simTime = 0.0
+        self.it = 0
+        self.itR = 0
+
+    def arr(self, name):
+        n = C.c_long()
+        p = self.L.oracle_array(self.h, OR[name], C.byref(n))
+        ct = C.c_float if self.bits == 32 else C.c_double
+        flat = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n.value,))
+        n3 = self.shape3[0] * self.shape3[1] * self.shape3[2]
+        if n.value >= n3 and n.value % n3 == 0:
+            return flat.reshape((n.value // n3,) + self.shape3)
+        return flat.reshape(self.shape3[:2])
+
+    def load_from_model(self, m):
+        """Copy the model's host state, base state, grid metrics and surface fields."""
+        self.arr("HYDRO")[...] = m.array("hydroFlds")
+        if m.array("sgstkeScalars") is not None:
+            self.arr("TKE")[: m.array("sgstkeScalars").shape[0]] = m.array("sgstkeScalars")
+        if m.array("moistScalars") is not None:
+            self.arr("MOIST")[: m.array("moistScalars").shape[0]] = m.array("moistScalars")
+        self.arr("BS")[...] = m.array("hydroBaseStateFlds")
+        for a, b in [("ZPOS", "zPos"), ("J31", "J31"), ("J32", "J32"), ("J33", "J33"), ("DJAC", "D_Jac"), ("INVDJAC", "invD_Jac")]:
+            self.arr(a)[0] = m.array(b)
+        for a, b in [("CD", "cdFld"), ("CH", "chFld"), ("CQ", "cqFld"), ("FRICVEL", "fricVel"), ("HTFLUX", "htFlux"),
+                     ("TSKIN", "tskin"), ("INVOBLEN", "invOblen"), ("Z0M", "z0m"), ("Z0T", "z0t"), ("QFLUX", "qFlux"),
+                     ("QSKIN", "qskin")]:
+            x = m.array(b)
+            if x is not None:
+                self.arr(a)[...] = x
+        self.simTime = m.getf("simTime")
+        self.it = m.geti("simTime_it")
+        self.itR = m.geti("simTime_itRestart")
+
+    def build_frhs(self, stage):
+        self.L.oracle_build_frhs(self.h, C.c_float(self.simTime), self.it, self.itR, int(stage))
+
+    def rk_stage(self, stage):
+        self.L.oracle_rk_stage(self.h, int(stage))
+
+    def steps(self, n):
+        st = C.c_float(self.simTime)
+        it = C.c_int(self.it)
+        self.L.oracle_steps(self.h, int(n), C.byref(st), C.byref(it), self.itR)
+        self.simTime, self.it = st.value, it.value
+
+    def apply_bcs(self):
+        self.L.oracle_apply_bcs(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def turbulent_state(m_or_arrays, seed=1, amp_u=1.0, amp_th=0.5, e_range=(0.05, 0.5), nh=3):
+    """Add smooth seeded random-phase perturbations to a model's host state in place.
+
+    Velocity perturbations of ~amp_u m/s, theta ~amp_th K, and rho*e = rho*U(e_range)
+    in the interior; halos are left to the boundary conditions.
+    """
+    hydro, tke = m_or_arrays
+    rng = np.random.default_rng(seed)
+    _, nxh, nyh, nzh = hydro.shape
+    nx, ny, nz = nxh - 2 * nh, nyh - 2 * nh, nzh - 2 * nh
+    x = np.arange(nx)[:, None, None]
+    y = np.arange(ny)[None, :, None]
+    z = np.arange(nz)[None, None, :]
+    sl = (slice(nh, nh + nx), slice(nh, nh + ny), slice(nh, nh + nz))
+    rho = hydro[0][sl].astype(np.float64)
+
+    def wave(a):
+        f = np.zeros((nx, ny, nz))
+        for _ in range(4):
+            kx, ky = rng.integers(1, 4, size=2)
+            kz = rng.uniform(0.5, 2.5)
+            ph = rng.uniform(0, 2 * np.pi, size=3)
+            f += np.sin(2 * np.pi * kx * x / nx + ph[0]) * np.sin(2 * np.pi * ky * y / ny + ph[1]) * np.sin(np.pi * kz * (z + 0.5) / nz + ph[2])
+        return a * f / 2.0
+
+    for fi, a in ((1, amp_u), (2, amp_u), (3, 0.5 * amp_u)):
+        hydro[fi][sl] = (hydro[fi][sl] + rho * wave(a)).astype(hydro.dtype)
+    hydro[4][sl] = (hydro[4][sl] + rho * wave(amp_th)).astype(hydro.dtype)
+    if tke is not None:
+        tke[0][sl] = (rho * rng.uniform(e_range[0], e_range[1], size=(nx, ny, nz))).astype(tke.dtype)

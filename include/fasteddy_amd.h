@@ -1,0 +1,75 @@
+/*
+ * fasteddy_amd.h -- C-ABI of the MI355X (gfx950) FastEddy time-step library.
+ *
+ * The entry points below replace the reference's CUDA layer one for one.
+ * They keep the reference's names, signatures, argument meanings and return
+ * convention (0 = success); inputs are the host modules' process globals and
+ * host arrays (halo-inclusive, z fastest), exactly as the reference's CUDA
+ * layer reads them.  Differences: errors are returned as non-zero codes
+ * instead of exit(), and only a single rank (numProcsX = numProcsY = 1) is
+ * accepted, with lateral periodicity applied on the device.
+ */
+#ifndef FASTEDDY_AMD_H
+#define FASTEDDY_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (non-zero returns). */
+#define FE_OK                 0
+#define FE_ERR_NO_DEVICE      101
+#define FE_ERR_HIP            102
+#define FE_ERR_ALLOC          103
+#define FE_ERR_UNSUPPORTED    104   /* a selector combination this build does not implement */
+#define FE_ERR_TERRAIN        105   /* metric tensor not flat (J31/J32 != 0 or not horizontally uniform) */
+#define FE_ERR_STATE          106   /* call order violated (e.g. Commence before Init) */
+#define FE_ERR_RANKS          107   /* numProcsX*numProcsY != 1 */
+
+/* SRC/FECUDA/fecuda.h:35-64 (fecuda.c:60-118): device selection and launch geometry. */
+int fecuda_Init(void);
+int fecuda_Cleanup(void);
+int fecuda_SetBlocksPerGrid(int Nx, int Ny, int Nz);        /* halo-inclusive extents, recorded */
+int fecuda_AllocateHaloBuffers(int Nxp, int Nyp, int Nzp, int Nh); /* single rank: no-op */
+int fecuda_DeallocateHaloBuffers(void);
+
+/* SRC/GRID/CUDA/cuda_grid.c:24,36: upload of the vertical coordinate and metric profiles. */
+int cuda_gridInit(void);
+int cuda_gridCleanup(void);
+
+/* SRC/HYDRO_CORE/CUDA/cuda_hydroCore.h:30,35: physics constants, base state, device fields. */
+int cuda_hydroCoreInit(void);
+int cuda_hydroCoreCleanup(void);
+
+/* SRC/TIME_INTEGRATION/CUDA/cuda_timeInt.h:30,35,40:
+ * Init allocates the RK stage buffers and copies the initial state to the device;
+ * Commence(it) runs NtBatch RK3 steps (uploading the host state again when
+ * it == simTime_itRestart), advances simTime/simTime_it, and copies the state,
+ * perturbation pressure and surface fields back to the host arrays. */
+int cuda_timeIntInit(void);
+int cuda_timeIntCommence(int it);
+int cuda_timeIntCleanup(void);
+
+/* ---- Build-only additions (no counterpart in the reference) ------------------ */
+
+/* Tendency of one RK stage from the current device state, written into the host
+ * arrays hydroFldsFrhs / sgstkeScalarsFrhs / moistScalarsFrhs (interior cells,
+ * halos zero).  The fused kernels never store F in normal operation; this is the
+ * parity hook.  'stage' is the 0-based RK stage; simTime/simTime_it as in
+ * cuda_hydroCoreDeviceBuildFrhs (SRC/HYDRO_CORE/CUDA/cuda_hydroCoreDevice.cu:246). */
+int fe_debug_build_frhs(int stage);
+/* Copy host state arrays to the device (same as the first-batch upload). */
+int fe_upload_state(void);
+/* Copy the device state back into the host arrays without stepping. */
+int fe_download_state(void);
+/* Device step time of the last Commence, in ms per step (HIP events). */
+float fe_last_ms_per_step(void);
+/* Run nsteps RK3 steps on the device without any host transfer (benchmarking). */
+int fe_device_steps(int nsteps);
+/* Name of the kernel variant selected for the stage-1/2 and stage-3 tendencies. */
+const char *fe_kernel_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

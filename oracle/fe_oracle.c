@@ -1,0 +1,1091 @@
+/*
+ * fe_oracle.c -- CPU oracle for the FastEddy compressible-LES time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fe_oracle.h).  A from-scratch restatement of
+ * the reference's per-stage tendency build ("stack C" of
+ * SRC/HYDRO_CORE/CUDA/cuda_hydroCoreDevice.cu:246-426) and of the RK3-WS2002
+ * stage update (SRC/TIME_INTEGRATION/CUDA/cuda_RKschemes.cu:21-157), written as
+ * serial phase loops over the halo-inclusive domain.
+ *
+ * Defined semantics where the reference's CUDA kernels race:
+ *   - boundary conditions are applied as three complete, ordered passes:
+ *     vertical ABL conditions, then the x-periodic copy, then the y-periodic
+ *     copy (cuda_hydroCoreDevice.cu:493-504 run per thread in that order);
+ *   - the SGS length scale is computed over its whole range before the eddy
+ *     diffusivity reads it at k-1 ("fresh" values, cuda_sgsTurbDevice.cu:237).
+ * Every other phase is a pure map, so serial evaluation is exactly the
+ * reference's arithmetic.
+ *
+ * Compile with -DREAL=float for the FP32 oracle (float storage, double
+ * literals promote exactly as in the reference sources) or -DREAL=double for
+ * the FP64 restatement.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "fe_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef REAL
+#define REAL float
+#endif
+typedef REAL real;
+
+#define IS_F32 (sizeof(real) == sizeof(float))
+/* libm entry points matching the storage precision (powf for FP32 as in the
+ * reference kernels, pow for the FP64 restatement). */
+static inline real r_pow(real a, real b)  { return IS_F32 ? (real)powf((float)a, (float)b) : (real)pow((double)a, (double)b); }
+static inline real r_sqrt(real a)         { return IS_F32 ? (real)sqrtf((float)a) : (real)sqrt((double)a); }
+static inline real r_log(real a)          { return IS_F32 ? (real)logf((float)a) : (real)log((double)a); }
+static inline real r_atan(real a)         { return IS_F32 ? (real)atanf((float)a) : (real)atan((double)a); }
+static inline real r_sin(real a)          { return IS_F32 ? (real)sinf((float)a) : (real)sin((double)a); }
+static inline real r_fmax(real a, real b) { return IS_F32 ? (real)fmaxf((float)a, (float)b) : (real)fmax((double)a, (double)b); }
+static inline real r_fmin(real a, real b) { return IS_F32 ? (real)fminf((float)a, (float)b) : (real)fmin((double)a, (double)b); }
+
+enum { RHO = 0, U = 1, V = 2, W = 3, TH = 4 };
+enum { T11 = 0, T21 = 1, T31 = 2, T32 = 3, T22 = 4, T33 = 5, TH1 = 6, TH2 = 7, TH3 = 8 };
+
+typedef struct {
+  fe_cfg c;
+  int Nh, iMin, iMax, jMin, jMax, kMin, kMax, NXH, NYH, NZH;
+  long N3, N2;
+  long sI, sJ;                   /* strides of i and j in a 3-D field */
+  real dX, dY, dZ, dXi, dYi, dZi;
+  real accel_g, R_gas, R_vapor, Rv_Rg, cv_gas, cp_gas, L_v, R_cp, cp_R, cp_cv, refPressure, kappa;
+  real corioConstHorz, corioConstVert, corioLS_fact;
+  real b_hyb, tke_b_hyb, qv_b, c_s, c_k, dt;
+  real U_g, V_g, z_Ug, z_Vg, Ug_grad, Vg_grad;
+  real dampDepth, filt_coeff;
+  real sl_z0, sl_z0t, sl_wth, sl_tr, sl_wq, sl_qr, sl_ts, sl_te, sl_amp, sl_qts, sl_qte, sl_qamp;
+  real temp_grnd, pres_grnd, cond_tscale, mp_tscale;
+  int nTKE, nMoist;
+  void *arr[OR_NUM_ARRAYS];
+  long cnt[OR_NUM_ARRAYS];
+} Oracle;
+
+#define ID3(o, i, j, k) ((long)(i) * (o)->sI + (long)(j) * (o)->sJ + (long)(k))
+#define ID2(o, i, j)    ((long)(i) * (o)->NYH + (long)(j))
+#define A(o, id) ((real *)(o)->arr[id])
+
+static int g_threads = 1;
+void oracle_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int oracle_real_size(void) { return (int)sizeof(real); }
+
+void *oracle_create(const fe_cfg *cfg) {
+  Oracle *o = (Oracle *)calloc(1, sizeof(Oracle));
+  if (!o) return NULL;
+  o->c = *cfg;
+  const fe_cfg *c = cfg;
+  o->Nh = c->Nh;
+  o->iMin = c->Nh; o->iMax = c->Nx + c->Nh;
+  o->jMin = c->Nh; o->jMax = c->Ny + c->Nh;
+  o->kMin = c->Nh; o->kMax = c->Nz + c->Nh;
+  o->NXH = c->Nx + 2 * c->Nh; o->NYH = c->Ny + 2 * c->Nh; o->NZH = c->Nz + 2 * c->Nh;
+  o->N3 = (long)o->NXH * o->NYH * o->NZH;
+  o->N2 = (long)o->NXH * o->NYH;
+  o->sJ = o->NZH; o->sI = (long)o->NYH * o->NZH;
+  /* Grid constants as set in SRC/GRID/grid.c:155-163 (float globals). */
+  o->dX = (real)(float)c->d_xi; o->dY = (real)(float)c->d_eta; o->dZ = (real)(float)c->d_zeta;
+  o->dXi = (real)(1.0 / o->dX); o->dYi = (real)(1.0 / o->dY); o->dZi = (real)(1.0 / o->dZ);
+  /* Physical constants, SRC/HYDRO_CORE/hydro_core.c:941-967. */
+  o->accel_g = (real)9.81f; o->R_gas = (real)287.04f; o->R_vapor = (real)461.60f;
+  o->cv_gas = (real)718.0f; o->cp_gas = o->R_gas + o->cv_gas; o->L_v = (real)2.5e6f;
+  o->R_cp = o->R_gas / o->cp_gas; o->cp_R = o->cp_gas / o->R_gas; o->cp_cv = o->cp_gas / o->cv_gas;
+  o->refPressure = (real)1.0e5f; o->Rv_Rg = o->R_vapor / o->R_gas; o->kappa = (real)0.4f;
+  {
+    real pi = IS_F32 ? (real)(float)acos(-1.0) : (real)acos(-1.0);
+    if (c->coriolisSelector > 0) {
+      o->corioConstHorz = (real)(1.45842e-4 * sin(pi / 180.0 * (real)c->coriolisLatitude));
+      o->corioConstVert = c->coriolisSelector > 1 ? (real)(1.45842e-4 * cos(pi / 180.0 * (real)c->coriolisLatitude)) : 0;
+      o->corioLS_fact = 1;
+    }
+  }
+  o->b_hyb = (real)c->b_hyb; o->tke_b_hyb = (real)c->TKEAdvSelector_b_hyb; o->qv_b = (real)c->moistureAdvSelectorQv_b;
+  o->c_s = (real)c->c_s; o->c_k = (real)c->c_k; o->dt = (real)c->dt;
+  o->U_g = (real)c->U_g; o->V_g = (real)c->V_g; o->z_Ug = (real)c->z_Ug; o->z_Vg = (real)c->z_Vg;
+  o->Ug_grad = (real)c->Ug_grad; o->Vg_grad = (real)c->Vg_grad;
+  o->dampDepth = (real)c->dampingLayerDepth; o->filt_coeff = (real)c->filter_6th_coeff;
+  o->sl_z0 = (real)c->surflayer_z0; o->sl_z0t = (real)c->surflayer_z0t; o->sl_wth = (real)c->surflayer_wth;
+  o->sl_tr = (real)c->surflayer_tr; o->sl_wq = (real)c->surflayer_wq; o->sl_qr = (real)c->surflayer_qr;
+  o->sl_ts = (real)c->surflayer_ideal_ts; o->sl_te = (real)c->surflayer_ideal_te; o->sl_amp = (real)c->surflayer_ideal_amp;
+  o->sl_qts = (real)c->surflayer_ideal_qts; o->sl_qte = (real)c->surflayer_ideal_qte; o->sl_qamp = (real)c->surflayer_ideal_qamp;
+  o->temp_grnd = (real)c->temp_grnd; o->pres_grnd = (real)c->pres_grnd;
+  o->cond_tscale = (real)c->moistureCondTscale; o->mp_tscale = (real)c->moistureMPcallTscale;
+  o->nTKE = (c->turbulenceSelector > 0) ? c->TKESelector : 0;
+  o->nMoist = (c->moistureSelector > 0) ? c->moistureNvars : 0;
+
+  long nf[OR_NUM_ARRAYS];
+  for (int i = 0; i < OR_NUM_ARRAYS; i++) nf[i] = 0;
+  nf[OR_HYDRO] = 5; nf[OR_HYDRO_F] = 5;
+  nf[OR_TKE] = o->nTKE ? o->nTKE : 1; nf[OR_TKE_F] = nf[OR_TKE];
+  nf[OR_MOIST] = o->nMoist ? o->nMoist : 1; nf[OR_MOIST_F] = nf[OR_MOIST];
+  nf[OR_BS] = 2;
+  nf[OR_ZPOS] = nf[OR_J31] = nf[OR_J32] = nf[OR_J33] = nf[OR_DJAC] = nf[OR_INVDJAC] = 1;
+  nf[OR_PRES] = nf[OR_RHOINV] = 1; nf[OR_FACEVEL] = 3; nf[OR_TAU] = 9;
+  nf[OR_KM] = 1; nf[OR_LS] = nf[OR_TKE]; nf[OR_DEDXI] = 3 * nf[OR_TKE];
+  nf[OR_MOISTTAU] = 3 * nf[OR_MOIST]; nf[OR_FCOND] = 1;
+  nf[OR_TIME0] = 5 + o->nTKE + o->nMoist;
+  for (int i = 0; i < OR_NUM_ARRAYS; i++) {
+    long n = (i >= OR_CD) ? o->N2 : nf[i] * o->N3;
+    o->cnt[i] = n;
+    o->arr[i] = calloc((size_t)n, sizeof(real));
+    if (!o->arr[i]) { oracle_destroy(o); return NULL; }
+  }
+  return o;
+}
+
+void oracle_destroy(void *h) {
+  Oracle *o = (Oracle *)h;
+  if (!o) return;
+  for (int i = 0; i < OR_NUM_ARRAYS; i++) free(o->arr[i]);
+  free(o);
+}
+
+void *oracle_array(void *h, int id, long *count) {
+  Oracle *o = (Oracle *)h;
+  if (!o || id < 0 || id >= OR_NUM_ARRAYS) return NULL;
+  if (count) *count = o->cnt[id];
+  return o->arr[id];
+}
+
+/* ------------------------------------------------------------------------- */
+/* Boundary conditions: SRC/HYDRO_CORE/CUDA/cuda_BCsDevice.cu:67-241.          */
+
+/* Geostrophic momentum base state, cuda_BCsDevice.cu:183-211. */
+static real momentum_bs(const Oracle *o, int fld, real z, real rho) {
+  real zg, vg, gr;
+  if (fld == U)      { zg = o->z_Ug; vg = o->U_g; gr = o->Ug_grad; }
+  else if (fld == V) { zg = o->z_Vg; vg = o->V_g; gr = o->Vg_grad; }
+  else               { zg = 0; vg = 0; gr = 0; }
+  if (z < zg) return vg * rho;
+  return (vg + gr * (z - zg)) * rho;
+}
+
+/* kind: 0 = base-state halos (rho, rho*theta), 1 = momentum (zero gradient below,
+ * geostrophic above), 2 = zero gradient below / value 'topval' above (SGS TKE),
+ * 3 = zero gradient below and above (moisture). */
+static void bc_vertical(Oracle *o, real *f, const real *bs, int kind, int fld) {
+  const real *z = A(o, OR_ZPOS);
+  const real *rbs = A(o, OR_BS);
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = 0; i < o->NXH; i++)
+    for (int j = 0; j < o->NYH; j++) {
+      long c0 = ID3(o, i, j, 0);
+      for (int k = 0; k < o->kMin; k++) {
+        if (kind == 0) f[c0 + k] = bs[c0 + k];
+        else f[c0 + k] = f[c0 + o->kMin];
+      }
+      for (int k = o->kMax; k < o->NZH; k++) {
+        if (kind == 0) f[c0 + k] = bs[c0 + k];
+        else if (kind == 1) f[c0 + k] = momentum_bs(o, fld, z[c0 + o->kMax - 1], rbs[c0 + o->kMax - 1]);
+        else if (kind == 2) f[c0 + k] = 0;
+        else f[c0 + k] = f[c0 + o->kMax - 1];
+      }
+    }
+}
+
+static void bc_periodic(Oracle *o, real *f) {
+  const int Nh = o->Nh;
+  /* x first, over every j and k (cuda_BCsDevice.cu:67-90) */
+  for (int i = 0; i < Nh; i++) {
+    memcpy(&f[ID3(o, i, 0, 0)], &f[ID3(o, o->iMax - Nh + i, 0, 0)], sizeof(real) * o->sI);
+    memcpy(&f[ID3(o, o->iMax + i, 0, 0)], &f[ID3(o, o->iMin + i, 0, 0)], sizeof(real) * o->sI);
+  }
+  /* then y, over every i (including x halos) and k (cuda_BCsDevice.cu:93-116) */
+  for (int i = 0; i < o->NXH; i++)
+    for (int j = 0; j < Nh; j++) {
+      memcpy(&f[ID3(o, i, j, 0)], &f[ID3(o, i, o->jMax - Nh + j, 0)], sizeof(real) * o->sJ);
+      memcpy(&f[ID3(o, i, o->jMax + j, 0)], &f[ID3(o, i, o->jMin + j, 0)], sizeof(real) * o->sJ);
+    }
+}
+
+int oracle_apply_bcs(void *h) {
+  Oracle *o = (Oracle *)h;
+  real *hy = A(o, OR_HYDRO);
+  real *bs = A(o, OR_BS);
+  for (int f = 0; f < 5; f++) {
+    real *fld = hy + f * o->N3;
+    if (f == U || f == V || f == W) bc_vertical(o, fld, NULL, 1, f);
+    else bc_vertical(o, fld, bs + (f == RHO ? 0 : 1) * o->N3, 0, f);
+    bc_periodic(o, fld);
+  }
+  for (int f = 0; f < o->nTKE; f++) {
+    real *fld = A(o, OR_TKE) + f * o->N3;
+    bc_vertical(o, fld, NULL, 2, 0);
+    bc_periodic(o, fld);
+  }
+  for (int f = 0; f < o->nMoist; f++) {
+    real *fld = A(o, OR_MOIST) + f * o->N3;
+    bc_vertical(o, fld, NULL, 3, 0);
+    bc_periodic(o, fld);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rho^-1 and perturbation pressure: cuda_hydroCoreDevice.cu:558-589,952-957;  */
+/* cuda_pressureDevice.cu:53-113.                                              */
+static void rhoinv_pressure(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *rth = A(o, OR_HYDRO) + TH * o->N3;
+  const real *rthbs = A(o, OR_BS) + o->N3;
+  const real *qv = A(o, OR_MOIST);
+  real *ri = A(o, OR_RHOINV), *p = A(o, OR_PRES);
+  const real eps = (real)1e-6f;
+  for (long n = 0; n < o->N3; n++) ri[n] = (real)(1.0 / r_fmax(rho[n], eps));
+  const real c1 = o->R_gas / r_pow(o->refPressure, o->R_cp);
+  const int moist = (o->c.moistureSelector > 0 && o->c.moistureNvars > 0);
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = 0; i < o->NXH; i++)
+    for (int j = 0; j < o->NYH; j++)
+      for (int k = 0; k < o->NZH; k++) {
+        int kt = k;
+        if (k > 0 && k < o->kMin) kt = o->kMin;
+        else if (k >= o->kMax && k < o->kMax + o->Nh) kt = o->kMax - 1;
+        long n = ID3(o, i, j, k), t = ID3(o, i, j, kt);
+        if (moist) {
+          real rthm = (real)(rth[t] * (1.0 + o->Rv_Rg * qv[t] / rho[t] * 1e-3));
+          p[n] = r_pow(rthm * c1, o->cp_cv) - r_pow(rthbs[t] * c1, o->cp_cv);
+        } else {
+          p[n] = r_pow(rth[t] * c1, o->cp_cv) - r_pow(rthbs[t] * c1, o->cp_cv);
+        }
+      }
+}
+
+/* Cell-face advecting velocities, cuda_advectionDevice.cu:54-121. */
+static void face_velocities(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *u = rho + U * o->N3, *v = rho + V * o->N3, *w = rho + W * o->N3;
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33), *D = A(o, OR_DJAC);
+  real *ucf = A(o, OR_FACEVEL), *vcf = ucf + o->N3, *wcf = ucf + 2 * o->N3;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i <= o->iMax; i++)
+    for (int j = o->jMin; j <= o->jMax; j++)
+      for (int k = o->kMin; k <= o->kMax; k++) {
+        long n = ID3(o, i, j, k), im = n - o->sI, jm = n - o->sJ, km = n - 1;
+        ucf[n] = (real)(0.5 * o->dXi * ((D[n] / rho[n]) * (u[n] + w[n] * J31[n]) + (D[im] / rho[im]) * (u[im] + w[im] * J31[im])));
+        vcf[n] = (real)(0.5 * o->dYi * ((D[n] / rho[n]) * (v[n] + w[n] * J32[n]) + (D[jm] / rho[jm]) * (v[jm] + w[jm] * J32[jm])));
+        wcf[n] = (real)(0.5 * o->dZi * ((D[n] / rho[n]) * (w[n] * J33[n]) + (D[km] / rho[km]) * (w[km] * J33[km])));
+        if (k == o->kMin || k >= o->kMax) wcf[n] = 0;
+      }
+}
+
+/* Pressure gradient force, cuda_pressureDevice.cu:118-194. */
+static void pressure_gradient(Oracle *o) {
+  const real *p = A(o, OR_PRES), *rho = A(o, OR_HYDRO), *mq = A(o, OR_MOIST);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  real *Fu = A(o, OR_HYDRO_F) + U * o->N3, *Fv = A(o, OR_HYDRO_F) + V * o->N3, *Fw = A(o, OR_HYDRO_F) + W * o->N3;
+  const int moist = (o->c.moistureSelector > 0 && o->c.moistureNvars > 0);
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        long ip = n + o->sI, im = n - o->sI, jp = n + o->sJ, jm = n - o->sJ, kp = n + 1, km = n - 1;
+        if (moist) {
+          real rmd = 1.0f;
+          for (int f = 0; f < o->c.moistureNvars; f++) rmd = (real)(rmd + mq[f * o->N3 + n] / rho[n] * 1e-3);
+          real rdm = (real)(1.0 / rmd);
+          Fu[n] = (real)(Fu[n] - rdm * 0.5 * o->dXi * (p[ip] - p[im]));
+          Fv[n] = (real)(Fv[n] - rdm * 0.5 * o->dYi * (p[jp] - p[jm]));
+          Fw[n] = (real)(Fw[n] - rdm * 0.5 * (o->dXi * J31[n] * (p[ip] - p[im]) + o->dYi * J32[n] * (p[jp] - p[jm]) + o->dZi * J33[n] * (p[kp] - p[km])));
+        } else {
+          Fu[n] = (real)(Fu[n] - 0.5 * o->dXi * (p[ip] - p[im]));
+          Fv[n] = (real)(Fv[n] - 0.5 * o->dYi * (p[jp] - p[jm]));
+          Fw[n] = (real)(Fw[n] - 0.5 * (o->dXi * J31[n] * (p[ip] - p[im]) + o->dYi * J32[n] * (p[jp] - p[jm]) + o->dZi * J33[n] * (p[kp] - p[km])));
+        }
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+/* SGS turbulence: cuda_sgsTurbDevice.cu:60-547, cuda_sgstkeDevice.cu:96-397.  */
+
+/* Strain-rate tensor and theta gradients on faces/edges, cuda_sgsTurbDevice.cu:60-197. */
+static void strain_rates(Oracle *o) {
+  const real *u = A(o, OR_HYDRO) + U * o->N3, *v = A(o, OR_HYDRO) + V * o->N3;
+  const real *w = A(o, OR_HYDRO) + W * o->N3, *th = A(o, OR_HYDRO) + TH * o->N3;
+  const real *ri = A(o, OR_RHOINV);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  real *T = A(o, OR_TAU);
+  const long N3 = o->N3;
+  const real dXi = o->dXi, dYi = o->dYi, dZi = o->dZi;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), im = n - o->sI, jm = n - o->sJ, km = n - 1;
+        long imjm = im - o->sJ, imkm = im - 1, jmkm = jm - 1, imjmkm = imjm - 1;
+#define VL(f, m) ((f)[m] * ri[m])
+        T[T11 * N3 + n] = dXi * (VL(u, n) - VL(u, im));
+        T[T22 * N3 + n] = dYi * (VL(v, n) - VL(v, jm));
+        T[T33 * N3 + n] = J31[n] * dXi * (VL(w, n) - VL(w, im)) + J32[n] * dYi * (VL(w, n) - VL(w, jm)) + J33[n] * dZi * (VL(w, n) - VL(w, km));
+        T[T21 * N3 + n] = (real)(0.5 * (0.5 * (dXi * (VL(v, n) - VL(v, im)) + dXi * (VL(v, jm) - VL(v, imjm)))
+                                       + 0.5 * (dYi * (VL(u, n) - VL(u, jm)) + dYi * (VL(u, im) - VL(u, imjm)))));
+        T[T31 * N3 + n] = (real)(0.5 * (0.5 * (dXi * (VL(w, n) - VL(w, im)) + dXi * (VL(w, km) - VL(w, imkm)))
+                                       + 0.5 * (J31[n] * dXi * (VL(u, n) - VL(u, im)) + J31[n] * dXi * (VL(u, km) - VL(u, imkm)))
+                                       + 0.25 * (J32[n] * dYi * (VL(u, n) - VL(u, jm)) + J32[n] * dYi * (VL(u, im) - VL(u, imjm))
+                                                 + J32[n] * dYi * (VL(u, km) - VL(u, jmkm)) + J32[n] * dYi * (VL(u, imkm) - VL(u, imjmkm)))
+                                       + 0.5 * (J33[n] * dZi * (VL(u, n) - VL(u, km)) + J33[n] * dZi * (VL(u, im) - VL(u, imkm)))));
+        T[T32 * N3 + n] = (real)(0.5 * (0.5 * (dYi * (VL(w, n) - VL(w, jm)) + dYi * (VL(w, km) - VL(w, jmkm)))
+                                       + 0.25 * (J31[n] * dXi * (VL(v, n) - VL(v, im)) + J31[n] * dXi * (VL(v, jm) - VL(v, imjm))
+                                                 + J31[n] * dXi * (VL(v, km) - VL(v, imkm)) + J31[n] * dXi * (VL(v, jmkm) - VL(v, imjmkm)))
+                                       + 0.5 * (J32[n] * dYi * (VL(v, n) - VL(v, jm)) + J32[n] * dYi * (VL(v, km) - VL(v, jmkm)))
+                                       + 0.5 * (J33[n] * dZi * (VL(v, n) - VL(v, km)) + J33[n] * dZi * (VL(v, jm) - VL(v, jmkm)))));
+        T[TH1 * N3 + n] = dXi * (VL(th, n) - VL(th, im));
+        T[TH2 * N3 + n] = dYi * (VL(th, n) - VL(th, jm));
+        T[TH3 * N3 + n] = (J31[n] * dXi * (VL(th, n) - VL(th, im))) + (J32[n] * dYi * (VL(th, n) - VL(th, jm))) + (J33[n] * dZi * (VL(th, n) - VL(th, km)));
+      }
+}
+
+/* Scalar gradients on faces for moisture SGS, cuda_sgsTurbDevice.cu:372-398. */
+static void grad_scalar_to_faces(Oracle *o, const real *s, real *dSx, real *dSy, real *dSz) {
+  const real *ri = A(o, OR_RHOINV);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin - 1; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), im = n - o->sI, jm = n - o->sJ, km = n - 1;
+        dSx[n] = o->dXi * (s[n] * ri[n] - s[im] * ri[im]);
+        dSy[n] = o->dYi * (s[n] * ri[n] - s[jm] * ri[jm]);
+        dSz[n] = J31[n] * o->dXi * (s[n] * ri[n] - s[im] * ri[im]) + J32[n] * o->dYi * (s[n] * ri[n] - s[jm] * ri[jm])
+               + J33[n] * o->dZi * (s[n] * ri[n] - s[km] * ri[km]);
+      }
+}
+
+static inline real turb_delta(const Oracle *o, real D) {
+  return IS_F32 ? (real)powf((float)(o->dX * o->dY * o->dZ * D), (float)(1.0 / 3.0))
+                : (real)pow((double)(o->dX * o->dY * o->dZ * D), 1.0 / 3.0);
+}
+
+/* SGS TKE length scale, cuda_sgstkeDevice.cu:96-151. */
+static void length_scale(Oracle *o, const real *e_s, real *ls) {
+  const real *th = A(o, OR_HYDRO) + TH * o->N3, *ri = A(o, OR_RHOINV);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33), *D = A(o, OR_DJAC);
+  const real lmin = (real)1e-2f;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin - 1; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), ip = n + o->sI, im = n - o->sI, jp = n + o->sJ, jm = n - o->sJ, kp = n + 1, km = n - 1;
+        real dthdz = (real)(0.5 * ((J31[n] * o->dXi * (th[ip] * ri[ip] - th[im] * ri[im]))
+                                 + (J32[n] * o->dYi * (th[jp] * ri[jp] - th[jm] * ri[jm]))
+                                 + (J33[n] * o->dZi * (th[kp] * ri[kp] - th[km] * ri[km]))));
+        real N2 = (o->accel_g / (th[n] * ri[n])) * dthdz;
+        real len2 = turb_delta(o, D[n]);
+        real l;
+        if (N2 > 0) {
+          real tb = r_pow(N2, (real)-0.5);
+          real e = e_s[n] * ri[n];
+          real len1 = (real)(0.76 * r_pow(e, (real)0.5) * tb);
+          l = r_fmax(r_fmin(len1, len2), lmin);
+        } else {
+          l = len2;
+        }
+        ls[n] = l;
+      }
+}
+
+/* Eddy diffusivity Km, cuda_sgsTurbDevice.cu:202-254 (accumulates into Km). */
+static void eddy_diff(Oracle *o, const real *e_s, const real *ls) {
+  const real *T = A(o, OR_TAU), *th = A(o, OR_HYDRO) + TH * o->N3, *ri = A(o, OR_RHOINV), *D = A(o, OR_DJAC);
+  real *Km = A(o, OR_KM);
+  const long N3 = o->N3;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), km = n - 1;
+        real kij;
+        if (o->c.TKESelector > 0) {
+          real e0 = e_s[n] * ri[n], e1 = e_s[km] * ri[km];
+          kij = (real)(o->c_k * 0.5 * (ls[n] * r_pow(e0, (real)0.5) + ls[km] * r_pow(e1, (real)0.5)));
+        } else {
+          real delta = turb_delta(o, D[n]);
+          real invPr = 3.0f;
+          real dthdz = T[TH3 * N3 + n];
+          real t1 = (real)(0.5 * (T[T11 * N3 + n] * T[T11 * N3 + n] + T[T22 * N3 + n] * T[T22 * N3 + n] + T[T33 * N3 + n] * T[T33 * N3 + n])
+                          + T[T21 * N3 + n] * T[T21 * N3 + n] + T[T31 * N3 + n] * T[T31 * N3 + n] + T[T32 * N3 + n] * T[T32 * N3 + n]);
+          real N2 = o->accel_g / (th[n] * ri[n]) * dthdz;
+          real t2 = N2 * invPr;
+          kij = (o->c_s * delta) * (o->c_s * delta) * r_sqrt(r_fmax(0, t1 - r_fmax(0, t2)));
+        }
+        Km[n] = Km[n] + kij;
+      }
+}
+
+/* Centred SGS TKE gradients, cuda_sgstkeDevice.cu:309-343. */
+static void grad_tke(Oracle *o, const real *s, real *dx, real *dy, real *dz) {
+  const real *ri = A(o, OR_RHOINV);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin - 1; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), ip = n + o->sI, im = n - o->sI, jp = n + o->sJ, jm = n - o->sJ, kp = n + 1, km = n - 1;
+        dx[n] = (real)(0.5 * (o->dXi * (s[ip] * ri[ip] - s[im] * ri[im])));
+        dy[n] = (real)(0.5 * (o->dYi * (s[jp] * ri[jp] - s[jm] * ri[jm])));
+        dz[n] = (real)(0.5 * (J31[n] * o->dXi * (s[ip] * ri[ip] - s[im] * ri[im]) + J32[n] * o->dYi * (s[jp] * ri[jp] - s[jm] * ri[jm])
+                              + J33[n] * o->dZi * (s[kp] * ri[kp] - s[km] * ri[km])));
+      }
+}
+
+/* tau_ij with the prognostic-TKE deviatoric term, cuda_sgsTurbDevice.cu:308-367;
+ * Smagorinsky form cuda_sgsTurbDevice.cu:259-302. */
+static void taus(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *u = rho + U * o->N3, *v = rho + V * o->N3, *w = rho + W * o->N3;
+  const real *Km = A(o, OR_KM), *ls = A(o, OR_LS), *e_s = A(o, OR_TKE), *D = A(o, OR_DJAC);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  real *T = A(o, OR_TAU);
+  const long N3 = o->N3;
+  const int tke = o->c.TKESelector > 0;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k), im = n - o->sI, jm = n - o->sJ, km = n - 1;
+        real delta = turb_delta(o, D[n]);
+        real lsc = tke ? ls[n] : delta;
+        real invPr = (real)(1.0 + 2.0 * lsc / delta);
+        real r = rho[n], K = Km[n];
+        real dev = 0;
+        if (tke) {
+          real div = K * (o->dXi * (u[n] / rho[n] - u[im] / rho[im]) + o->dYi * (v[n] / rho[n] - v[jm] / rho[jm])
+                          + J31[n] * o->dXi * (w[n] / rho[n] - w[im] / rho[im]) + J32[n] * o->dYi * (w[n] / rho[n] - w[jm] / rho[jm])
+                          + J33[n] * o->dZi * (w[n] / rho[n] - w[km] / rho[km]))
+                   + e_s[n] / rho[n];
+          dev = (real)((2.0 / 3.0) * r * div);
+        }
+        T[T11 * N3 + n] = (real)(-2.0 * r * K * T[T11 * N3 + n] + dev);
+        T[T21 * N3 + n] = (real)(-2.0 * r * K * T[T21 * N3 + n]);
+        T[T31 * N3 + n] = (real)(-2.0 * r * K * T[T31 * N3 + n]);
+        T[T32 * N3 + n] = (real)(-2.0 * r * K * T[T32 * N3 + n]);
+        T[T22 * N3 + n] = (real)(-2.0 * r * K * T[T22 * N3 + n] + dev);
+        T[T33 * N3 + n] = (real)(-2.0 * r * K * T[T33 * N3 + n] + dev);
+        T[TH1 * N3 + n] = -r * K * invPr * T[TH1 * N3 + n];
+        T[TH2 * N3 + n] = -r * K * invPr * T[TH2 * N3 + n];
+        T[TH3 * N3 + n] = -r * K * invPr * T[TH3 * N3 + n];
+      }
+}
+
+/* Moisture SGS fluxes, cuda_sgsTurbDevice.cu:403-442. */
+static void taus_scalar(Oracle *o, real *M1, real *M2, real *M3) {
+  const real *rho = A(o, OR_HYDRO), *Km = A(o, OR_KM), *ls = A(o, OR_LS), *D = A(o, OR_DJAC);
+  const int tke = o->c.TKESelector > 0;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin - 1; i < o->iMax + 1; i++)
+    for (int j = o->jMin - 1; j < o->jMax + 1; j++)
+      for (int k = o->kMin; k < o->kMax + 1; k++) {
+        long n = ID3(o, i, j, k);
+        real delta = turb_delta(o, D[n]);
+        real lsc = tke ? ls[n] : delta;
+        real invPr = (real)(1.0 + 2.0 * lsc / delta);
+        M1[n] = -rho[n] * Km[n] * invPr * M1[n];
+        M2[n] = -rho[n] * Km[n] * invPr * M2[n];
+        M3[n] = -rho[n] * Km[n] * invPr * M3[n];
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Surface layer: cuda_surfaceLayerDevice.cu:112-521.                          */
+static void surface_layer(Oracle *o, float simTime, int it, int itR, int stage) {
+  const real *rho = A(o, OR_HYDRO), *u = rho + U * o->N3, *v = rho + V * o->N3, *th = rho + TH * o->N3;
+  const real *qv = A(o, OR_MOIST), *J33 = A(o, OR_J33);
+  real *T = A(o, OR_TAU), *MT = A(o, OR_MOISTTAU);
+  real *cd = A(o, OR_CD), *ch = A(o, OR_CH), *cq = A(o, OR_CQ), *ust = A(o, OR_FRICVEL);
+  real *hf = A(o, OR_HTFLUX), *tsk = A(o, OR_TSKIN), *iol = A(o, OR_INVOBLEN);
+  real *z0m = A(o, OR_Z0M), *z0t = A(o, OR_Z0T), *qf = A(o, OR_QFLUX), *qsk = A(o, OR_QSKIN);
+  const long N3 = o->N3;
+  const int moist = o->c.moistureSelector > 0;
+  const int sel = o->c.surflayerSelector;
+  const real pi = IS_F32 ? (real)acosf(-1.0f) : (real)acos(-1.0);
+  const real dt = o->dt;
+  const int tfreq = (int)roundf((float)(10.0 / dt));
+  const real tfreq_f = (real)tfreq;
+  const int k = o->kMin;
+  for (int i = o->iMin; i <= o->iMax + 1; i++)
+    for (int j = o->jMin; j <= o->jMax + 1; j++) {
+      long n = ID3(o, i, j, k), ij = ID2(o, i, j);
+      /* land-surface model (LSMdry :112-195, LSMmoist :199-318) */
+      real z0 = z0m[ij];
+      real z1 = (real)(0.5 / (o->dZi * J33[n]));
+      real z1oz0 = (z1 + z0) / z0;
+      real z0tm = z0t[ij];
+      real z1ozt0 = (z1 + z0tm) / z0tm;
+      real u1 = u[n] / rho[n], v1 = v[n] / rho[n];
+      real U1 = r_sqrt(r_pow(u1, 2) + r_pow(v1, 2));
+      real th1 = th[n] / rho[n];
+      real q1 = moist ? qv[n] / rho[n] : 0;
+      real th0;
+      if (it == itR && it == 0) {
+        cd[ij] = r_pow(o->kappa, 2) / r_pow(r_log(z1oz0), 2);
+        ch[ij] = r_pow(o->kappa, 2) / r_pow(r_log(z1ozt0), 2);
+        if (moist) cq[ij] = r_pow(o->kappa, 2) / r_pow(r_log(z1ozt0), 2);
+      } else if (it == itR && it != 0) {
+        cd[ij] = r_pow(ust[ij], 2) / r_pow(U1, 2);
+        th0 = tsk[ij] * r_pow(o->refPressure / o->pres_grnd, o->R_cp);
+        ch[ij] = hf[ij] / (U1 * (th0 - th1));
+        if (moist) cq[ij] = qf[ij] / (U1 * (qsk[ij] - q1));
+      }
+      if (sel == 1) {
+        if (o->c.surflayer_idealsine == 1) {
+          if ((simTime > o->sl_ts) && (simTime <= o->sl_te))
+            hf[ij] = (real)(o->sl_wth + o->sl_amp * sin(pi * (simTime - o->sl_ts) / (o->sl_te - o->sl_ts)));
+          else
+            hf[ij] = o->sl_wth;
+          if (moist) {
+            if ((simTime > o->sl_qts) && (simTime <= o->sl_qte))
+              qf[ij] = (real)(o->sl_qamp * sin(pi * (simTime - o->sl_qts) / (o->sl_qte - o->sl_qts)));
+            else
+              qf[ij] = o->sl_wq;
+          }
+        }
+      } else if (sel == 2) {
+        int upd = it % tfreq;
+        real tskc, qskc = 0;
+        if (upd == 0 && stage == 2) {
+          if (o->c.surflayer_idealsine == 1 && (simTime > o->sl_ts) && (simTime < o->sl_te + dt * tfreq_f)) {
+            real prev = simTime - tfreq_f * dt;
+            prev = r_fmax(prev, o->sl_ts);
+            real inc = (real)(o->sl_amp * (sin(pi * (simTime - o->sl_ts) / (o->sl_te - o->sl_ts)) - sin(pi * (prev - o->sl_ts) / (o->sl_te - o->sl_ts))));
+            tskc = tsk[ij] + inc;
+            tsk[ij] = tskc;
+            if (moist) {
+              real qinc = (real)(o->sl_qamp * (sin(pi * (simTime - o->sl_qts) / (o->sl_qte - o->sl_qts)) - sin(pi * (prev - o->sl_qts) / (o->sl_qte - o->sl_qts))));
+              qskc = qsk[ij] + qinc;
+              qsk[ij] = qskc;
+            }
+          } else {
+            tskc = (real)(tsk[ij] + o->sl_tr * dt * tfreq_f / 3600.0);
+            tsk[ij] = tskc;
+            if (moist) { qskc = (real)(qsk[ij] + o->sl_qr * dt * tfreq_f / 3600.0); qsk[ij] = qskc; }
+          }
+        } else {
+          tskc = tsk[ij];
+          if (moist) qskc = qsk[ij];
+        }
+        th0 = tskc * r_pow(o->refPressure / o->pres_grnd, o->R_cp);
+        hf[ij] = ch[ij] * U1 * (th0 - th1);
+        if (moist) qf[ij] = cq[ij] * U1 * (qskc - q1);
+      }
+      /* Monin-Obukhov iteration (MOSTdry :322-414, MOSTmoist :418-521) */
+      {
+        real beta = 5.0f, ol_lim = 1.0f;
+        int it_max = (o->c.surflayer_stab == 0) ? 5 : 1;
+        real cd_i = 0, ch_i = 0, cq_i = 0, cd_0 = cd[ij];
+        real tauxz, tauyz, zol, xi, psi_m, psi_h;
+        th0 = tsk[ij] * r_pow(o->refPressure / o->pres_grnd, o->R_cp);
+        real q0 = moist ? qsk[ij] : 0;
+        int itn = 0;
+        do {
+          tauxz = -cd_0 * U1 * u1;
+          tauyz = -cd_0 * U1 * v1;
+          ust[ij] = r_pow(r_pow(tauxz, 2) + r_pow(tauyz, 2), (real)0.25);
+          itn = itn + 1;
+          if (o->c.surflayer_stab == 0) {
+            if (ust[ij] > 0) {
+              iol[ij] = -(o->kappa * o->accel_g * hf[ij]) / (r_pow(ust[ij], 3) * th1);
+              iol[ij] = r_fmax(r_fmin(iol[ij], ol_lim), -ol_lim);
+            } else {
+              iol[ij] = -ol_lim;
+            }
+          } else {
+            iol[ij] = 0;
+          }
+          zol = iol[ij] * (z1 + z0);
+          if (zol < 0) {
+            xi = (real)r_pow((real)(1.0 - 16.0 * zol), (real)0.25);
+            psi_m = (real)(r_log((real)(0.5 * (1.0 + r_pow(xi, 2)) * r_pow((real)(0.5 * (1.0 + xi)), 2))) - 2.0 * r_atan(xi) + 0.5 * pi);
+            psi_h = (real)(2.0 * r_log((real)(0.5 * (1.0 + r_pow(xi, 2)))));
+          } else {
+            psi_m = -beta * zol;
+            psi_h = -beta * zol;
+          }
+          cd_i = r_pow(o->kappa, 2) / r_pow(r_log(z1oz0) - psi_m, 2);
+          ch_i = r_pow(o->kappa, 2) / ((r_log(z1ozt0) - psi_m) * (r_log(z1ozt0) - psi_h));
+          if (moist) cq_i = r_pow(o->kappa, 2) / ((r_log(z1ozt0) - psi_m) * (r_log(z1ozt0) - psi_h));
+          if (sel > 1) {
+            hf[ij] = ch_i * U1 * (th0 - th1);
+            if (moist) qf[ij] = cq_i * U1 * (q0 - q1);
+          }
+          cd_0 = cd_i;
+        } while (itn <= it_max);
+        cd[ij] = cd_i;
+        ch[ij] = ch_i;
+        if (moist) cq[ij] = cq_i;
+        real Um = r_sqrt(r_pow(u[n] / rho[n], 2) + r_pow(v[n] / rho[n], 2));
+        tauxz = -cd_i * Um * u[n];
+        tauyz = -cd_i * Um * v[n];
+        T[T31 * N3 + n] = tauxz;
+        T[T32 * N3 + n] = tauyz;
+        ust[ij] = r_pow(r_pow(tauxz, 2) + r_pow(tauyz, 2), (real)0.25);
+        T[TH3 * N3 + n] = hf[ij] * rho[n];
+        if (moist) MT[2 * N3 + n] = qf[ij] * rho[n];
+        iol[ij] = -(o->kappa * o->accel_g * hf[ij]) / (r_pow(ust[ij], 3) * th1);
+      }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Advection: cuda_advectionDevice.cu:123-926.  Each scheme is a face value at */
+/* the low face (i-1/2) of cell i, from phi[i-3..i+2] (p[0..5]) and the sign   */
+/* of the face velocity; the divergence is u(i+1)*f(i+1/2) - u(i)*f(i-1/2).    */
+static inline real copysign1(real u) { return signbit(u) ? (real)-1 : (real)1; }
+
+static real weno5_side(real a, real b, real c, real d, real e) {
+  /* upwind reconstruction at the face from cells a..e with e nearest the face's
+   * downwind side: fh from stencils (a,b,c), (b,c,d), (c,d,e)
+   * cuda_advectionDevice.cu:643-663 */
+  const real g1 = (real)(1.0 / 10.0), g2 = (real)(3.0 / 5.0), g3 = (real)(3.0 / 10.0), tol = (real)1e-6f;
+  real fh1 = (real)((1.0 / 3.0) * a - (7.0 / 6.0) * b + (11.0 / 6.0) * c);
+  real fh2 = (real)(-(1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d);
+  real fh3 = (real)((1.0 / 3.0) * c + (5.0 / 6.0) * d - (1.0 / 6.0) * e);
+  real b1 = (real)((13.0 / 12.0) * r_pow((real)(a - 2.0 * b + c), 2) + (1.0 / 4.0) * r_pow((real)(a - 4.0 * b + 3.0 * c), 2));
+  real b2 = (real)((13.0 / 12.0) * r_pow((real)(b - 2.0 * c + d), 2) + (1.0 / 4.0) * r_pow((real)(b - d), 2));
+  real b3 = (real)((13.0 / 12.0) * r_pow((real)(c - 2.0 * d + e), 2) + (1.0 / 4.0) * r_pow((real)(3.0 * c - 4.0 * d + e), 2));
+  real w1t = g1 / r_pow(tol + b1, 2), w2t = g2 / r_pow(tol + b2, 2), w3t = g3 / r_pow(tol + b3, 2);
+  real w1 = w1t / (w1t + w2t + w3t), w2 = w2t / (w1t + w2t + w3t), w3 = w3t / (w1t + w2t + w3t);
+  return w1 * fh1 + w2 * fh2 + w3 * fh3;
+}
+
+static real weno3_side(real a, real b, real c) {
+  /* WENO3, cuda_advectionDevice.cu:383-470: face value from (a,b) and (b,c) */
+  const real g1 = (real)(1.0 / 3.0), g2 = (real)(2.0 / 3.0), tol = (real)1e-6f;
+  real fh1 = (real)(-0.5 * a + 1.5 * b);
+  real fh2 = (real)(0.5 * b + 0.5 * c);
+  real b1 = r_pow(b - a, 2), b2 = r_pow(c - b, 2);
+  real w1t = g1 / r_pow(tol + b1, 2), w2t = g2 / r_pow(tol + b2, 2);
+  real w1 = w1t / (w1t + w2t), w2 = w2t / (w1t + w2t);
+  return w1 * fh1 + w2 * fh2;
+}
+
+/* Returns the advective flux through the low face: u * f(i-1/2).  p[0..5] = phi[i-3..i+2]. */
+static real face_flux(int scheme, real b, real u, const real *p) {
+  real m3 = p[0], m2 = p[1], m1 = p[2], c0 = p[3], p1 = p[4], p2 = p[5];
+  switch (scheme) {
+    case 1: /* QUICK (:209-243) */
+      return (real)((1.0 / 8.0) * (r_fmax(0, u) * (6.0 * m1 + 3.0 * c0 - m2) + r_fmin(0, u) * (6.0 * c0 + 3.0 * m1 - p1)));
+    case 2: { /* hybrid 3rd/4th (:248-313) */
+      real f = (real)((real)(1.0 / 12.0) * (7.0 * (c0 + m1) - (p1 + m2) + (1.0 - b) * copysign1(u) * ((p1 - m2) - 3.0 * (c0 - m1))));
+      return u * f;
+    }
+    case 3: { /* hybrid 5th/6th (:318-380) */
+      real f = (real)((real)(1.0 / 60.0) * (37.0 * (c0 + m1) - 8.0 * (p1 + m2) + (p2 + m3)
+                                           - (1.0 - b) * copysign1(u) * ((p2 - m3) - 5.0 * (p1 - m2) + 10.0 * (c0 - m1))));
+      return u * f;
+    }
+    case 4: { /* WENO3 (:383-537) */
+      real fp = weno3_side(m2, m1, c0), fn = weno3_side(p1, c0, m1);
+      real f = (real)(0.5 * (1.0 + copysign1(u)) * fp + 0.5 * (1.0 - copysign1(u)) * fn);
+      return u * f;
+    }
+    case 5: { /* WENO5 (:628-926) */
+      real fp = weno5_side(m3, m2, m1, c0, p1), fn = weno5_side(p2, p1, c0, m1, m2);
+      real f = (real)(0.5 * (1.0 + copysign1(u)) * fp + 0.5 * (1.0 - copysign1(u)) * fn);
+      return u * f;
+    }
+    case 6: /* centred 2nd (:147-176) */
+      return u * (real)(0.5 * (c0 + m1));
+    default: /* 1st-order upstream (:126-145) */
+      return r_fmax(0, u) * m1 + r_fmin(0, u) * c0;
+  }
+}
+
+static void advect(Oracle *o, const real *fld, real *F, int scheme, real b) {
+  const real *ucf = A(o, OR_FACEVEL), *vcf = ucf + o->N3, *wcf = ucf + 2 * o->N3;
+  const real *invD = A(o, OR_INVDJAC);
+  const long sI = o->sI, sJ = o->sJ;
+  const int split = (scheme == 4 || scheme == 5); /* WENO accumulates per direction */
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        real px[7], py[7], pz[7];
+        for (int q = 0; q < 7; q++) { px[q] = fld[n + (q - 3) * sI]; py[q] = fld[n + (q - 3) * sJ]; pz[q] = fld[n + (q - 3)]; }
+        real dx = face_flux(scheme, b, ucf[n + sI], px + 1) - face_flux(scheme, b, ucf[n], px);
+        real dy = face_flux(scheme, b, vcf[n + sJ], py + 1) - face_flux(scheme, b, vcf[n], py);
+        real dz = face_flux(scheme, b, wcf[n + 1], pz + 1) - face_flux(scheme, b, wcf[n], pz);
+        if (split) {
+          F[n] = F[n] - invD[n] * dx;
+          F[n] = F[n] - invD[n] * dy;
+          F[n] = F[n] - invD[n] * dz;
+        } else if (scheme == 0 || scheme == 1) {
+          F[n] = F[n] - invD[n] * (dx + dy + dz);
+        } else {
+          F[n] = F[n] - invD[n] * (dx + dy + dz);
+        }
+      }
+}
+
+/* Rayleigh damping on rho*w, cuda_rayleighDampingDevice.cu:49-79. */
+static void rayleigh(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *w = rho + W * o->N3, *rbs = A(o, OR_BS), *z = A(o, OR_ZPOS);
+  real *Fw = A(o, OR_HYDRO_F) + W * o->N3;
+  const real pio2 = (real)(0.5 * acos(-1.0));
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k), top = ID3(o, i, j, o->kMax - 1);
+        if (z[n] >= (z[top] - o->dampDepth)) {
+          real wbs = momentum_bs(o, W, z[n], rbs[n]);
+          real s = r_sin((real)(pio2 * (1.0 - (z[top] - z[n]) / o->dampDepth)));
+          Fw[n] = (real)(Fw[n] - 0.2 * rho[n] * pow((double)s, 2) * (w[n] / rho[n] - wbs / rbs[n]));
+        }
+      }
+}
+
+/* Buoyancy on rho*w, cuda_buoyancyDevice.cu:48-67. */
+static void buoyancy(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *rbs = A(o, OR_BS), *mq = A(o, OR_MOIST);
+  real *Fw = A(o, OR_HYDRO_F) + W * o->N3;
+  const int nm = o->nMoist;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        if (nm == 1) {
+          real rv = mq[n];
+          Fw[n] = (real)(Fw[n] - (rho[n] / (rho[n] + rv * 1e-3)) * o->accel_g * ((rho[n] + rv * 1e-3) - rbs[n]));
+        } else if (nm == 2) {
+          real rv = mq[n], rl = mq[o->N3 + n];
+          Fw[n] = (real)(Fw[n] - (rho[n] / (rho[n] + rv * 1e-3 + rl * 1e-3)) * o->accel_g * ((rho[n] + rv * 1e-3 + rl * 1e-3) - rbs[n]));
+        } else {
+          Fw[n] = Fw[n] - o->accel_g * (rho[n] - rbs[n]);
+        }
+      }
+}
+
+/* f-plane Coriolis with large-scale geostrophic term, cuda_coriolisDevice.cu:50-58. */
+static void coriolis(Oracle *o) {
+  const real *rho = A(o, OR_HYDRO), *u = rho + U * o->N3, *v = rho + V * o->N3, *w = rho + W * o->N3;
+  const real *rbs = A(o, OR_BS), *z = A(o, OR_ZPOS);
+  real *F = A(o, OR_HYDRO_F);
+  const long N3 = o->N3;
+  const real fh = o->corioConstHorz, fv = o->corioConstVert, ls = o->corioLS_fact;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        real ub = momentum_bs(o, U, z[n], rbs[n]), vb = momentum_bs(o, V, z[n], rbs[n]), wb = momentum_bs(o, W, z[n], rbs[n]);
+        F[U * N3 + n] = F[U * N3 + n] + (fh * (v[n] / rho[n] - ls * vb / rbs[n]) - fv * (w[n] / rho[n] - ls * wb / rbs[n]));
+        F[V * N3 + n] = F[V * N3 + n] - (fh * (u[n] / rho[n] - ls * ub / rbs[n]));
+        F[W * N3 + n] = F[W * N3 + n] + (fv * (u[n] / rho[n] - ls * ub / rbs[n]));
+      }
+}
+
+/* Divergence of SGS stresses, cuda_sgsTurbDevice.cu:447-547. */
+static void turb_mixing(Oracle *o) {
+  const real *T = A(o, OR_TAU);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  real *F = A(o, OR_HYDRO_F);
+  const long N3 = o->N3, sI = o->sI, sJ = o->sJ;
+  const real dXi = o->dXi, dYi = o->dYi, dZi = o->dZi;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k), ip = n + sI, jp = n + sJ, kp = n + 1;
+        const real *t11 = T + T11 * N3, *t21 = T + T21 * N3, *t31 = T + T31 * N3, *t32 = T + T32 * N3;
+        const real *t22 = T + T22 * N3, *t33 = T + T33 * N3, *h1 = T + TH1 * N3, *h2 = T + TH2 * N3, *h3 = T + TH3 * N3;
+        F[U * N3 + n] = F[U * N3 + n] - (dXi * (t11[ip] - t11[n]) + dYi * (t21[jp] - t21[n]) + J31[n] * dXi * (t31[ip] - t31[n])
+                                         + J32[n] * dYi * (t31[jp] - t31[n]) + J33[n] * dZi * (t31[kp] - t31[n]));
+        F[V * N3 + n] = F[V * N3 + n] - (dXi * (t21[ip] - t21[n]) + dYi * (t22[jp] - t22[n]) + J31[n] * dXi * (t32[ip] - t32[n])
+                                         + J32[n] * dYi * (t32[jp] - t32[n]) + J33[n] * dZi * (t32[kp] - t32[n]));
+        F[W * N3 + n] = F[W * N3 + n] - (dXi * (t31[ip] - t31[n]) + dYi * (t32[jp] - t32[n]) + J31[n] * dXi * (t33[ip] - t33[n])
+                                         + J32[n] * dYi * (t33[jp] - t33[n]) + J33[n] * dZi * (t33[kp] - t33[n]));
+        F[TH * N3 + n] = F[TH * N3 + n] - (dXi * (h1[ip] - h1[n]) + dYi * (h2[jp] - h2[n]) + J31[n] * dXi * (h3[ip] - h3[n])
+                                           + J32[n] * dYi * (h3[jp] - h3[n]) + J33[n] * dZi * (h3[kp] - h3[n]));
+      }
+}
+
+static void turb_mixing_scalar(Oracle *o, real *Fm, const real *M1, const real *M2, const real *M3) {
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  const long sI = o->sI, sJ = o->sJ;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k), ip = n + sI, jp = n + sJ, kp = n + 1;
+        Fm[n] = Fm[n] - (o->dXi * (M1[ip] - M1[n]) + o->dYi * (M2[jp] - M2[n]) + J31[n] * o->dXi * (M3[ip] - M3[n])
+                         + J32[n] * o->dYi * (M3[jp] - M3[n]) + J33[n] * o->dZi * (M3[kp] - M3[n]));
+      }
+}
+
+/* Prognostic SGS TKE sources, cuda_sgstkeDevice.cu:68-92,155-397. */
+static void sgstke_sources(Oracle *o, int f) {
+  const real *rho = A(o, OR_HYDRO), *u = rho + U * o->N3, *v = rho + V * o->N3, *w = rho + W * o->N3, *th = rho + TH * o->N3;
+  const real *ri = A(o, OR_RHOINV), *T = A(o, OR_TAU), *Km = A(o, OR_KM), *D = A(o, OR_DJAC);
+  const real *J31 = A(o, OR_J31), *J32 = A(o, OR_J32), *J33 = A(o, OR_J33);
+  const real *es = A(o, OR_TKE) + f * o->N3, *ls = A(o, OR_LS) + f * o->N3;
+  const real *dex = A(o, OR_DEDXI) + (3 * f + 0) * o->N3, *dey = A(o, OR_DEDXI) + (3 * f + 1) * o->N3, *dez = A(o, OR_DEDXI) + (3 * f + 2) * o->N3;
+  real *Fe = A(o, OR_TKE_F) + f * o->N3;
+  const long N3 = o->N3, sI = o->sI, sJ = o->sJ;
+  const real dXi = o->dXi, dYi = o->dYi, dZi = o->dZi;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k), ip = n + sI, im = n - sI, jp = n + sJ, jm = n - sJ, kp = n + 1, km = n - 1;
+        /* buoyancy (:155-187) */
+        Fe[n] = Fe[n] + o->accel_g / (th[n] * ri[n]) * T[TH3 * N3 + n];
+        /* dissipation (:191-233), with the length-scale corrected c_e */
+        {
+          real delta = turb_delta(o, D[n]);
+          real ce = (real)(1.9 * o->c_k + (0.93 - 1.9 * o->c_k) * ls[n] / delta);
+          real e = es[n] * ri[n];
+          Fe[n] = Fe[n] + (-ce * es[n] * r_pow(e, (real)0.5) / ls[n]);
+        }
+        /* shear production (:237-305) */
+        {
+#define GX(f) (0.5 * (dXi * ((f)[ip] * ri[ip] - (f)[im] * ri[im])))
+#define GY(f) (0.5 * (dYi * ((f)[jp] * ri[jp] - (f)[jm] * ri[jm])))
+#define GZ(f) (0.5 * (J31[n] * dXi * ((f)[ip] * ri[ip] - (f)[im] * ri[im]) + J32[n] * dYi * ((f)[jp] * ri[jp] - (f)[jm] * ri[jm]) + J33[n] * dZi * ((f)[kp] * ri[kp] - (f)[km] * ri[km])))
+          real t11 = (real)(-T[T11 * N3 + n] * GX(u)), t12 = (real)(-T[T21 * N3 + n] * GY(u)), t13 = (real)(-T[T31 * N3 + n] * GZ(u));
+          real t21 = (real)(-T[T21 * N3 + n] * GX(v)), t22 = (real)(-T[T22 * N3 + n] * GY(v)), t23 = (real)(-T[T32 * N3 + n] * GZ(v));
+          real t31 = (real)(-T[T31 * N3 + n] * GX(w)), t32 = (real)(-T[T32 * N3 + n] * GY(w)), t33 = (real)(-T[T33 * N3 + n] * GZ(w));
+#undef GX
+#undef GY
+#undef GZ
+          Fe[n] = Fe[n] + (t11 + t12 + t13 + t21 + t22 + t23 + t31 + t32 + t33);
+        }
+        /* turbulent transport (:347-397) */
+        {
+          real tx = -(dXi * (dex[ip] * Km[ip] * rho[ip] - dex[im] * Km[im] * rho[im]));
+          real ty = -(dYi * (dey[jp] * Km[jp] * rho[jp] - dey[jm] * Km[jm] * rho[jm]));
+          real tz = -(J31[n] * dXi * (dez[ip] * Km[ip] * rho[ip] - dez[im] * Km[im] * rho[im])
+                      + J32[n] * dYi * (dez[jp] * Km[jp] * rho[jp] - dez[jm] * Km[jm] * rho[jm])
+                      + J33[n] * dZi * (dez[kp] * Km[kp] * rho[kp] - dez[km] * Km[km] * rho[km]));
+          Fe[n] = Fe[n] + (-(tx + ty + tz));
+        }
+      }
+}
+
+/* Saturation adjustment and latent heating, cuda_moistureDevice.cu:80-245. */
+static void microphysics(Oracle *o) {
+  const real *rth = A(o, OR_HYDRO) + TH * o->N3, *ri = A(o, OR_RHOINV), *rthbs = A(o, OR_BS) + o->N3;
+  const real *qv = A(o, OR_MOIST), *ql = A(o, OR_MOIST) + o->N3;
+  real *fc = A(o, OR_FCOND), *Fqv = A(o, OR_MOIST_F), *Fql = A(o, OR_MOIST_F) + o->N3, *Fth = A(o, OR_HYDRO_F) + TH * o->N3;
+  const real Tf = 273.15f;
+  const real a0 = 6.11239921f, a1 = 0.443987641f, a2 = 0.142986287e-1f, a3 = 0.264847430e-3f, a4 = 0.302950461e-5f;
+  const real a5 = 0.206739458e-7f, a6 = 0.640689451e-10f, a7 = -0.952447341e-13f, a8 = -0.976195544e-15f;
+  const real a0i = 6.11147274f, a1i = 0.503160820f, a2i = 0.188439774e-1f, a3i = 0.420895665e-3f, a4i = 0.615021634e-5f;
+  const real a5i = 0.602588177e-7f, a6i = 0.385852041e-9f, a7i = 0.146898966e-11f, a8i = 0.252751365e-14f;
+  const real c1 = o->R_gas / r_pow(o->refPressure, o->R_cp);
+  const int cond = o->c.moistureCond;
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        /* fcond (moistZerothOrder :98-190) */
+        real thd = rth[n] * ri[n];
+        real pr = (o->c.moistureCondBasePres == 1) ? r_pow(rthbs[n] * c1, o->cp_cv) : r_pow(rth[n] * c1, o->cp_cv);
+        real Td = thd * r_pow(pr / o->refPressure, o->R_cp);
+        real Tc = r_fmax((real)-80., (real)(Td - 273.16));
+        real pvs;
+        if (Td >= Tf)
+          pvs = (real)((a0 + Tc * (a1 + Tc * (a2 + Tc * (a3 + Tc * (a4 + Tc * (a5 + Tc * (a6 + Tc * (a7 + a8 * Tc)))))))) * 100.);
+        else
+          pvs = (real)((a0i + Tc * (a1i + Tc * (a2i + Tc * (a3i + Tc * (a4i + Tc * (a5i + Tc * (a6i + Tc * (a7i + a8i * Tc)))))))) * 100.);
+        pvs = r_fmin((real)(0.99 * pr), pvs);
+        real ffor = 0, flim = 0;
+        if (cond == 1) {
+          ffor = (real)(qv[n] * 1e-3 - (pvs / (o->R_vapor * Td)));
+          flim = (real)(ql[n] * 1e-3);
+        } else if (cond == 2) {
+          real rv = (real)(qv[n] * 1e-3), rvs = pvs / (o->R_vapor * Td);
+          ffor = (real)((rv - rvs) / (1.0 + (r_pow(o->L_v, 2) * rvs * ri[n] / (o->cp_gas * o->R_vapor * r_pow(Td, 2)))));
+          flim = (real)(ql[n] * 1e-3);
+        } else if (cond == 3) {
+          real rv = (real)(qv[n] * 1e-3);
+          real rvs = pvs / (pr - pvs) / o->Rv_Rg;
+          rvs = rvs / ri[n];
+          ffor = (real)((rv - rvs) / (1.0 + (r_pow(o->L_v, 2) * rvs * ri[n] / (o->cp_gas * o->R_vapor * r_pow(Td, 2)))));
+          flim = (real)(ql[n] * 1e-3);
+        }
+        real f = r_fmax(ffor, 0) - r_fmax(r_fmin(flim, -ffor), 0);
+        fc[n] = f / o->cond_tscale;
+      }
+  /* qv/ql forcing (moistCondFrhs :194-212) and latent heating (thetaCondFrhs :216-245) */
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        Fqv[n] = (real)(Fqv[n] - fc[n] * 1e3);
+        Fql[n] = (real)(Fql[n] + fc[n] * 1e3);
+      }
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        real thd = rth[n] * ri[n];
+        real pr = r_pow(rth[n] * c1, o->cp_cv);
+        real Td = thd * r_pow(pr / o->refPressure, o->R_cp);
+        Fth[n] = Fth[n] + ((thd * o->L_v) / (Td * o->cp_gas)) * fc[n];
+      }
+}
+
+/* 6th-order vertical filter on rho*w with a downgradient limiter, cuda_filtersDevice.cu:46-122. */
+static void filter_w(Oracle *o) {
+  const real *w = A(o, OR_HYDRO) + W * o->N3;
+  real *Fw = A(o, OR_HYDRO_F) + W * o->N3;
+  const real passes = 1.0f, c1 = 10.0f, c2 = 5.0f;
+  const real coeff = (real)(0.015625 * o->filt_coeff / (passes * o->dt));
+#pragma omp parallel for num_threads(g_threads)
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        long n = ID3(o, i, j, k);
+        real gm = w[n] - w[n - 1], gp = w[n + 1] - w[n];
+        real fm = c1 * (w[n] - w[n - 1]) - c2 * (w[n + 1] - w[n - 2]) + (w[n + 2] - w[n - 3]);
+        real fp = c1 * (w[n + 1] - w[n]) - c2 * (w[n + 2] - w[n - 1]) + (w[n + 3] - w[n - 2]);
+        gm = gm * fm;
+        gp = gp * fp;
+        fm = r_fmax(copysign1(gm), 0) * fm;
+        fp = r_fmax(copysign1(gp), 0) * fp;
+        Fw[n] = Fw[n] + coeff * (fp - fm);
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+int oracle_build_frhs(void *h, float simTime, int simTime_it, int simTime_itRestart, int stage) {
+  Oracle *o = (Oracle *)h;
+  const fe_cfg *c = &o->c;
+  const int numRKstages = 2;
+  const int physics = (c->physics_oneRKonly == 0) || (stage == numRKstages);
+  const long N3 = o->N3;
+
+  /* 1. boundary conditions and zeroed accumulators (Commence kernel :431-556) */
+  oracle_apply_bcs(o);
+  memset(A(o, OR_HYDRO_F), 0, sizeof(real) * 5 * N3);
+  memset(A(o, OR_TKE_F), 0, sizeof(real) * o->cnt[OR_TKE_F]);
+  memset(A(o, OR_MOIST_F), 0, sizeof(real) * o->cnt[OR_MOIST_F]);
+  if (c->turbulenceSelector > 0) memset(A(o, OR_KM), 0, sizeof(real) * N3);
+
+  /* 2. rho^-1 and perturbation pressure (:558-589) */
+  rhoinv_pressure(o);
+
+  /* 3. face velocities, PGF, SGS stresses, surface layer (:780-947) */
+  face_velocities(o);
+  if (c->pgfSelector > 0) pressure_gradient(o);
+  if (c->turbulenceSelector > 0 && physics) {
+    strain_rates(o);
+    if (o->nMoist > 0 && c->moistureSGSturb > 0)
+      for (int f = 0; f < o->nMoist; f++) {
+        real *MT = A(o, OR_MOISTTAU);
+        grad_scalar_to_faces(o, A(o, OR_MOIST) + f * N3, MT + (3 * f + 0) * N3, MT + (3 * f + 1) * N3, MT + (3 * f + 2) * N3);
+      }
+    if (c->TKESelector > 0) {
+      for (int f = 0; f < o->nTKE; f++) {
+        real *es = A(o, OR_TKE) + f * N3, *ls = A(o, OR_LS) + f * N3, *de = A(o, OR_DEDXI) + 3 * f * N3;
+        length_scale(o, es, ls);
+        eddy_diff(o, es, ls);
+        grad_tke(o, es, de, de + N3, de + 2 * N3);
+      }
+      taus(o);
+    } else {
+      eddy_diff(o, A(o, OR_TKE), A(o, OR_LS));
+      taus(o);
+    }
+    if (o->nMoist > 0 && c->moistureSGSturb > 0)
+      for (int f = 0; f < o->nMoist; f++) {
+        real *MT = A(o, OR_MOISTTAU);
+        taus_scalar(o, MT + (3 * f + 0) * N3, MT + (3 * f + 1) * N3, MT + (3 * f + 2) * N3);
+      }
+  }
+  if (c->surflayerSelector > 0 && physics) surface_layer(o, simTime, simTime_it, simTime_itRestart, stage);
+
+  /* 4. advection, damping, buoyancy, Coriolis, SGS divergence (:591-778) */
+  for (int f = 0; f < 5; f++) {
+    advect(o, A(o, OR_HYDRO) + f * N3, A(o, OR_HYDRO_F) + f * N3, c->advectionSelector, o->b_hyb);
+    if (f == W) {
+      if (c->dampingLayerSelector > 0) rayleigh(o);
+      if (c->buoyancySelector > 0) buoyancy(o);
+    }
+  }
+  if (c->turbulenceSelector > 0 && c->TKESelector > 0)
+    for (int f = 0; f < o->nTKE; f++)
+      advect(o, A(o, OR_TKE) + f * N3, A(o, OR_TKE_F) + f * N3, c->TKEAdvSelector, o->tke_b_hyb);
+  for (int f = 0; f < o->nMoist; f++) {
+    int sch;
+    if (f == 0) sch = c->moistureAdvSelectorQv;
+    else sch = (c->moistureAdvSelectorQi == 0) ? 0 : (c->moistureAdvSelectorQi == 1 ? 4 : 5);
+    advect(o, A(o, OR_MOIST) + f * N3, A(o, OR_MOIST_F) + f * N3, sch, o->qv_b);
+  }
+  if (c->coriolisSelector > 0) coriolis(o);
+  if (c->turbulenceSelector > 0 && physics) {
+    turb_mixing(o);
+    if (o->nMoist > 0 && c->moistureSGSturb > 0)
+      for (int f = 0; f < o->nMoist; f++) {
+        real *MT = A(o, OR_MOISTTAU);
+        turb_mixing_scalar(o, A(o, OR_MOIST_F) + f * N3, MT + (3 * f + 0) * N3, MT + (3 * f + 1) * N3, MT + (3 * f + 2) * N3);
+      }
+  }
+
+  /* 5.-6. SGS TKE sources and microphysics (:373-388) */
+  if (physics) {
+    if (c->turbulenceSelector > 0 && c->TKESelector > 0)
+      for (int f = 0; f < o->nTKE; f++) sgstke_sources(o, f);
+    if (c->moistureSelector > 0 && c->moistureCond > 0 && c->moistureNvars > 1) {
+      int tf = (int)roundf(fmaxf((float)o->mp_tscale, (float)o->dt) / (float)o->dt);
+      if (simTime_it % tf == 0) microphysics(o);
+    }
+  }
+
+  /* 7. explicit vertical filter (:414-416) */
+  if (c->filterSelector > 0) filter_w(o);
+  return 0;
+}
+
+/* RK3-WS2002 stage update, cuda_RKschemes.cu:21-157. */
+int oracle_rk_stage(void *h, int stage) {
+  Oracle *o = (Oracle *)h;
+  const long N3 = o->N3;
+  const int nT = o->nTKE, nM = o->nMoist;
+  real *t0 = A(o, OR_TIME0);
+  for (int g = 0; g < 5 + nT + nM; g++) {
+    real *phi, *F;
+    real floor_v = 0;
+    int clamp = 0;
+    if (g < 5) { phi = A(o, OR_HYDRO) + g * N3; F = A(o, OR_HYDRO_F) + g * N3; }
+    else if (g < 5 + nT) { phi = A(o, OR_TKE) + (g - 5) * N3; F = A(o, OR_TKE_F) + (g - 5) * N3; clamp = 1; floor_v = (real)1e-10f; }
+    else { phi = A(o, OR_MOIST) + (g - 5 - nT) * N3; F = A(o, OR_MOIST_F) + (g - 5 - nT) * N3; clamp = 1; floor_v = 0; }
+    real *p0 = t0 + g * N3;
+#pragma omp parallel for num_threads(g_threads)
+    for (int i = o->iMin; i < o->iMax; i++)
+      for (int j = o->jMin; j < o->jMax; j++)
+        for (int k = o->kMin; k < o->kMax; k++) {
+          long n = ID3(o, i, j, k);
+          if (stage == 0) {
+            p0[n] = phi[n];
+            phi[n] = (real)(phi[n] + (1.0 / 3.0) * o->dt * F[n]);
+          } else if (stage == 1) {
+            phi[n] = (real)(p0[n] + 0.5 * o->dt * F[n]);
+          } else {
+            phi[n] = p0[n] + o->dt * F[n];
+          }
+          if (clamp) phi[n] = r_fmax(phi[n], floor_v);
+        }
+  }
+  return 0;
+}
+
+int oracle_steps(void *h, int nsteps, float *simTime, int *simTime_it, int simTime_itRestart) {
+  Oracle *o = (Oracle *)h;
+  for (int s = 0; s < nsteps; s++) {
+    for (int st = 0; st < 3; st++) {
+      oracle_build_frhs(o, *simTime, *simTime_it, simTime_itRestart, st);
+      oracle_rk_stage(o, st);
+    }
+    *simTime = *simTime + (float)o->dt;
+    *simTime_it = *simTime_it + 1;
+  }
+  return 0;
+}

@@ -1,0 +1,101 @@
+/*
+ * fe_oracle.h -- CPU oracle for the FastEddy compressible-LES time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker: a plain-C restatement of the
+ * reference's HYDRO_CORE tendency build and TIME_INTEGRATION RK3-WS2002 stage
+ * update, used by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg.  Nothing in the product path links or calls it.
+ *
+ * The same source is compiled twice: with REAL=float (the reference's FP32
+ * storage, with the reference's double-literal promotions kept) and with
+ * REAL=double (an FP64 restatement of the same discrete problem, used to
+ * measure FP32 rounding error).
+ *
+ * Arrays use the reference layout: halo-inclusive, z fastest,
+ *   ijk = i*(Ny+2Nh)*(Nz+2Nh) + j*(Nz+2Nh) + k
+ * (SRC/HYDRO_CORE/CUDA/cuda_hydroCoreDevice.cu:571-573), 2-D surface arrays
+ *   ij = i*(Ny+2Nh) + j  (SRC/HYDRO_CORE/hydro_core.c:839).
+ */
+#ifndef FE_ORACLE_H
+#define FE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Primary parameters, as read from a .in file (float values carried in doubles).
+ * Field meanings follow SRC/HYDRO_CORE/hydro_core.c:197-395 and
+ * SRC/GRID/grid.c:68-88, SRC/TIME_INTEGRATION/time_integration.c:51-64. */
+typedef struct {
+  int Nx, Ny, Nz, Nh;
+  double d_xi, d_eta, d_zeta;
+  int hydroBCs;
+  int advectionSelector;   double b_hyb;
+  int TKEAdvSelector;      double TKEAdvSelector_b_hyb;
+  int pgfSelector, buoyancySelector, coriolisSelector;
+  double coriolisLatitude;
+  int turbulenceSelector, TKESelector;
+  double c_s, c_k;
+  int surflayerSelector;
+  double surflayer_z0, surflayer_z0t, surflayer_wth, surflayer_tr, surflayer_wq, surflayer_qr;
+  int surflayer_idealsine;
+  double surflayer_ideal_ts, surflayer_ideal_te, surflayer_ideal_amp;
+  double surflayer_ideal_qts, surflayer_ideal_qte, surflayer_ideal_qamp;
+  int surflayer_stab;
+  int moistureSelector, moistureNvars, moistureAdvSelectorQv;
+  double moistureAdvSelectorQv_b;
+  int moistureAdvSelectorQi, moistureSGSturb, moistureCond;
+  double moistureCondTscale;
+  int moistureCondBasePres;
+  double moistureMPcallTscale;
+  int filterSelector;      double filter_6th_coeff;
+  int dampingLayerSelector; double dampingLayerDepth;
+  double temp_grnd, pres_grnd;
+  double U_g, V_g, z_Ug, z_Vg, Ug_grad, Vg_grad;
+  int physics_oneRKonly;
+  double dt;
+} fe_cfg;
+
+/* Array identifiers for oracle_array(). */
+enum {
+  OR_HYDRO = 0,     /* 5 fields: rho, rho*u, rho*v, rho*w, rho*theta        */
+  OR_HYDRO_F,       /* 5 tendency fields                                      */
+  OR_TKE,           /* TKESelector fields (rho*e)                             */
+  OR_TKE_F,
+  OR_MOIST,         /* moistureNvars fields (rho*qv, rho*ql in g/kg)          */
+  OR_MOIST_F,
+  OR_BS,            /* 2 fields: rho_BS, (rho*theta)_BS                       */
+  OR_ZPOS, OR_J31, OR_J32, OR_J33, OR_DJAC, OR_INVDJAC,
+  OR_PRES,          /* perturbation pressure                                  */
+  OR_RHOINV,
+  OR_FACEVEL,       /* 3 fields                                               */
+  OR_TAU,           /* 9 fields: 11,21,31,32,22,33,th1,th2,th3                */
+  OR_KM, OR_LS, OR_DEDXI /* 3 fields */,
+  OR_MOISTTAU,      /* 3*moistureNvars fields                                 */
+  OR_FCOND,
+  OR_TIME0,         /* stage-0 copy of all prognostic fields                  */
+  /* 2-D surface arrays */
+  OR_CD, OR_CH, OR_CQ, OR_FRICVEL, OR_HTFLUX, OR_TSKIN, OR_INVOBLEN,
+  OR_Z0M, OR_Z0T, OR_QFLUX, OR_QSKIN,
+  OR_NUM_ARRAYS
+};
+
+void *oracle_create(const fe_cfg *cfg);
+void  oracle_destroy(void *h);
+/* Pointer to an oracle-owned, zero-initialised array; *count receives the element count. */
+void *oracle_array(void *h, int id, long *count);
+int   oracle_real_size(void);
+void  oracle_set_threads(int n);
+/* One tendency build (reference stack C), leaving F in the *_F arrays. */
+int   oracle_build_frhs(void *h, float simTime, int simTime_it, int simTime_itRestart, int stage);
+/* One RK3-WS2002 stage update using the current F. */
+int   oracle_rk_stage(void *h, int stage);
+/* nsteps full RK3 steps (3 x (build_frhs + rk_stage)); advances *simTime and *simTime_it. */
+int   oracle_steps(void *h, int nsteps, float *simTime, int *simTime_it, int simTime_itRestart);
+/* Apply the stage-start boundary conditions to the state arrays only. */
+int   oracle_apply_bcs(void *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

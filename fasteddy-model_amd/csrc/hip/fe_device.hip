@@ -56,9 +56,25 @@ struct Dev {
   float ms_per_step = 0;
   int gridBlocks[3] = {0, 0, 0};
   int tb[3] = {0, 0, 0};
-  hipGraphExec_t graph[2] = {nullptr, nullptr};   /* [0]: step without microphysics, [1]: with */
-  int graph_ok = 0;
+  /* per-kernel HIP-event timing on the launch stream: class 0 column pass, 1 plain stage, 2 physics stage */
+  int timing = 0;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> evcls;
+  size_t evn = 0;
+  float *probe = nullptr;
+  long probe_n = 0;
 } D;
+
+void timed_begin(int cls) {
+  if (!D.timing || D.evn + 2 > D.ev.size()) return;
+  hipEventRecord(D.ev[D.evn], D.stream);
+  D.evcls[D.evn / 2] = cls;
+}
+void timed_end(void) {
+  if (!D.timing || D.evn + 2 > D.ev.size()) return;
+  hipEventRecord(D.ev[D.evn + 1], D.stream);
+  D.evn += 2;
+}
 
 enum { SF_CD = 0, SF_CH, SF_CQ, SF_UST, SF_HF, SF_TSK, SF_IOL, SF_Z0M, SF_Z0T, SF_QF, SF_QSK, SF_T13, SF_T23, SF_TH3, SF_TQ3, SF_N };
 
@@ -200,10 +216,14 @@ int launch_stage(int stage, const float *in, const float *base, float *out, int 
   const int phys = physics_stage(stage) && turbulenceSelector > 0;
   if (physics_stage(stage) && (phys || surflayerSelector > 0)) {
     dim3 cg((D.Nx + 255) / 256, D.Ny);
+    timed_begin(0);
     hipLaunchKernelGGL(g_var.column, cg, dim3(256), 0, D.stream, a, phys, surflayerSelector > 0);
+    timed_end();
   }
   dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY);
+  timed_begin(phys ? 2 : 1);
   hipLaunchKernelGGL(phys ? g_var.phys : g_var.plain, grid, dim3(NT), 0, D.stream, a);
+  timed_end();
   HIPCHK(hipGetLastError());
   return FE_OK;
 }
@@ -528,6 +548,69 @@ int fe_device_steps(int nsteps) {
   }
   HIPCHK(hipStreamSynchronize(D.stream));
   return FE_OK;
+}
+
+/* Per-kernel timing: enable with a pool for npairs launches; read the summed
+ * milliseconds and launch counts per kernel class after a synchronising call. */
+int fe_timing_enable(int npairs) {
+  for (auto e : D.ev) hipEventDestroy(e);
+  D.ev.clear();
+  D.evcls.clear();
+  D.evn = 0;
+  D.timing = npairs > 0;
+  if (!D.timing) return FE_OK;
+  D.ev.resize((size_t)2 * npairs);
+  D.evcls.assign((size_t)npairs, -1);
+  for (auto &e : D.ev) HIPCHK(hipEventCreate(&e));
+  return FE_OK;
+}
+
+int fe_timing_read(double *ms, int *count, int ncls) {
+  for (int c = 0; c < ncls; ++c) { ms[c] = 0; count[c] = 0; }
+  HIPCHK(hipStreamSynchronize(D.stream));
+  for (size_t i = 0; i + 1 < D.evn; i += 2) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, D.ev[i], D.ev[i + 1]));
+    int c = D.evcls[i / 2];
+    if (c >= 0 && c < ncls) { ms[c] += t; count[c] += 1; }
+  }
+  D.evn = 0;
+  return FE_OK;
+}
+
+__global__ void k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, long n4) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n4; i += stride) b[i] = a[i];
+}
+
+/* Streaming copy of nfloats floats, reps times, timed with HIP events; returns GB/s
+ * (read + write bytes).  Used to calibrate the HBM byte counters and to report the
+ * box's measured copy bandwidth. */
+double fe_copy_probe(long nfloats, int reps) {
+  nfloats = (nfloats + 3) / 4 * 4;
+  if (D.probe_n < nfloats) {
+    if (D.probe) hipFree(D.probe);
+    if (hipMalloc(&D.probe, (size_t)2 * nfloats * sizeof(float)) != hipSuccess) return -1;
+    hipMemset(D.probe, 0, (size_t)2 * nfloats * sizeof(float));
+    D.probe_n = nfloats;
+  }
+  if (!D.stream) hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking);
+  const long n4 = nfloats / 4;
+  const float4 *a = (const float4 *)D.probe;
+  float4 *b = (float4 *)(D.probe + nfloats);
+  int blocks = 256 * 8;
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, D.stream, a, b, n4);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0); hipEventCreate(&e1);
+  hipEventRecord(e0, D.stream);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, D.stream, a, b, n4);
+  hipEventRecord(e1, D.stream);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  return (double)2 * nfloats * sizeof(float) * reps / (ms * 1e-3) / 1e9;
 }
 
 int fe_debug_build_frhs(int stage) {

@@ -79,6 +79,7 @@ class Model:
     """One model instance over the product library (process-global state: one at a time)."""
 
     _lib = None
+    SINGLE_3D = {"hydroPres", "hydroBaseStatePres", "xPos", "yPos", "zPos", "J31", "J32", "J33", "D_Jac", "invD_Jac"}
 
     @classmethod
     def lib(cls):
@@ -159,6 +160,8 @@ class Model:
         s3 = self.shape3
         n3 = s3[0] * s3[1] * s3[2]
         n2 = s3[0] * s3[1]
+        if name in self.SINGLE_3D:
+            return flat.reshape(s3)
         if n.value % n3 == 0 and n.value >= n3:
             return flat.reshape((n.value // n3,) + s3)
         return flat.reshape(self.shape2)

@@ -66,6 +66,14 @@ int fe_download_state(void);
 float fe_last_ms_per_step(void);
 /* Run nsteps RK3 steps on the device without any host transfer (benchmarking). */
 int fe_device_steps(int nsteps);
+/* Per-kernel HIP-event timing on the launch stream (classes: 0 column pass,
+ * 1 stage kernel without SGS physics, 2 stage kernel with SGS physics).
+ * fe_timing_enable(npairs) arms a pool for npairs launches (0 disarms);
+ * fe_timing_read synchronises and returns summed ms and launch counts. */
+int fe_timing_enable(int npairs);
+int fe_timing_read(double *ms, int *count, int ncls);
+/* Streaming float4 copy of nfloats floats, reps times; returns GB/s (read+write). */
+double fe_copy_probe(long nfloats, int reps);
 /* Name of the kernel variant selected for the stage-1/2 and stage-3 tendencies. */
 const char *fe_kernel_info(void);
 

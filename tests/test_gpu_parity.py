@@ -1,0 +1,138 @@
+"""GPU parity of the fused gfx950 stage kernels against the CPU oracle (through the C-ABI).
+
+Tolerance (single-stage tendencies, per prognostic field):
+    max|F_gpu - F_64| <= max(1e-5 * max|F_64|, 2 * max|F_ref32 - F_64|)
+where F_64 is the FP64 oracle and F_ref32 the FP32 oracle, which keeps the
+reference's FP32 arithmetic.  That is: the GPU tendency is within 1e-5 of the
+field maximum of the exact-arithmetic answer, or at least as accurate (to a
+factor 2) as the reference's own FP32 evaluation.  The reference's
+difference-of-powers perturbation pressure puts its FP32 momentum tendencies
+3-7e-5 off FP64, so the second branch is what the reference itself meets; the
+GPU's perturbation-form pressure stays near 3e-7 there.
+"""
+import numpy as np
+import pytest
+
+import fe_py
+
+pytestmark = pytest.mark.gpu
+NH = 3
+
+
+def rel_err(a, b):
+    ax = tuple(range(1, a.ndim))
+    return np.abs(a - b).max(axis=ax) / np.maximum(np.abs(b).max(axis=ax), 1e-300)
+
+
+def check_tendencies(m, o32, o64, stage, label):
+    m.build_frhs(stage)
+    o32.build_frhs(stage)
+    o64.build_frhs(stage)
+    pairs = [("hydroFldsFrhs", "HYDRO_F", 5)]
+    if m.array("sgstkeScalarsFrhs") is not None:
+        pairs.append(("sgstkeScalarsFrhs", "TKE_F", m.array("sgstkeScalarsFrhs").shape[0]))
+    if m.array("moistScalarsFrhs") is not None:
+        pairs.append(("moistScalarsFrhs", "MOIST_F", m.array("moistScalarsFrhs").shape[0]))
+    for hname, oname, nf in pairs:
+        g = m.interior(m.array(hname)).astype(np.float64)
+        r32 = m.interior(o32.arr(oname)[:nf]).astype(np.float64)
+        r64 = m.interior(o64.arr(oname)[:nf])
+        e_gpu = rel_err(g, r64)
+        e_ref = rel_err(r32, r64)
+        print(f"{label} stage {stage} {hname}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
+        assert np.all(np.isfinite(g))
+        assert np.all(e_gpu <= np.maximum(1e-5, 2 * e_ref)), (hname, e_gpu, e_ref)
+
+
+def make(params, tmp_path, seed):
+    m = fe_py.Model(params, str(tmp_path))
+    fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=seed)
+    m.upload()
+    o32, o64 = fe_py.Oracle(params, 32), fe_py.Oracle(params, 64)
+    o32.load_from_model(m)
+    o64.load_from_model(m)
+    return m, o32, o64
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 20), (70, 18, 24), (128, 8, 10)])
+def test_stage_tendencies_dry(shape, tmp_path):
+    nx, ny, nz = shape
+    p = dict(fe_py.CFG0, Nx=nx, Ny=ny, Nz=nz)
+    m, o32, o64 = make(p, tmp_path, seed=nx)
+    try:
+        for stage in (0, 1, 2):
+            check_tendencies(m, o32, o64, stage, f"{shape}")
+    finally:
+        m.close()
+
+
+def test_stage_tendencies_filter_and_stretched_grid(tmp_path):
+    p = dict(fe_py.CFG0, Nx=32, Ny=16, Nz=30, filterSelector=1, filter_6th_coeff=0.03,
+             verticalDeformSwitch=1, verticalDeformFactor=0.7, dampingLayerDepth=150.0)
+    m, o32, o64 = make(p, tmp_path, seed=4)
+    try:
+        for stage in (0, 2):
+            check_tendencies(m, o32, o64, stage, "filter+stretched")
+    finally:
+        m.close()
+
+
+def test_geostrophic_rest_gpu(tmp_path):
+    p = dict(fe_py.CFG0, Nx=64, Ny=8, Nz=16, thetaPerturbationSwitch=0, surflayerSelector=0)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        m.build_frhs(0)
+        F = m.interior(m.array("hydroFldsFrhs"))
+        # advection, buoyancy and pressure terms vanish exactly; Coriolis to rounding
+        assert np.all(F[[0, 1, 3, 4]] == 0.0)
+        assert np.abs(F[2]).max() < 1e-9
+    finally:
+        m.close()
+
+
+def test_mass_conservation_gpu(tmp_path):
+    p = dict(fe_py.CFG0, Nx=64, Ny=16, Nz=20)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=9)
+        m.upload()
+        for stage in (0, 2):
+            m.build_frhs(stage)
+            Fr = m.interior(m.array("hydroFldsFrhs"))[0].astype(np.float64)
+            assert abs(Fr.sum()) <= 1e-6 * np.abs(Fr).sum()
+    finally:
+        m.close()
+
+
+def test_rk3_steps_match_oracle(tmp_path):
+    """Full RK3 steps (tendencies, stage updates, TKE clamp, surface-layer state) through
+    cuda_timeIntCommence against the FP64 oracle: after 1 and 10 steps the state difference is a
+    small fraction of the state change."""
+    p = dict(fe_py.CFG0, Nx=64, Ny=16, Nz=24, NtBatch=1)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=21)
+        o64 = fe_py.Oracle(p, 64)
+        o64.load_from_model(m)
+        h0 = m.interior(m.array("hydroFlds")).astype(np.float64).copy()
+        e0 = m.interior(m.array("sgstkeScalars")).astype(np.float64).copy()
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        done = 0
+        for target in (1, 10):
+            while done < target:
+                m.commence()
+                done += 1
+            o64.steps(target - (o64.it))
+            hg = m.interior(m.array("hydroFlds")).astype(np.float64)
+            ho = m.interior(o64.arr("HYDRO"))
+            change = np.abs(ho - h0).max(axis=(1, 2, 3))
+            diff = np.abs(hg - ho).max(axis=(1, 2, 3))
+            print(f"after {target} steps: |gpu-fp64| {diff} vs |change| {change}")
+            assert np.all(diff <= 1e-3 * change + 1e-6 * np.abs(ho).max(axis=(1, 2, 3))), (diff, change)
+            eg = m.interior(m.array("sgstkeScalars")).astype(np.float64)
+            eo = m.interior(o64.arr("TKE")[:1])
+            assert np.abs(eg - eo).max() <= 1e-3 * np.abs(eo - e0).max() + 1e-7
+            np.testing.assert_allclose(m.array("fricVel")[NH:-NH, NH:-NH], o64.arr("FRICVEL")[NH:-NH, NH:-NH], rtol=1e-4)
+        assert m.geti("simTime_it") == 10
+    finally:
+        m.close()

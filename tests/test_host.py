@@ -1,0 +1,196 @@
+"""Host layer: .in grammar and validation, glibc-compatible IC random sequence,
+grid metrics, base state and initial conditions, and the C-ABI exports.
+
+These run without a GPU (host modules only; no device entry point is called).
+"""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fe_py
+
+NH = 3
+
+
+def lib():
+    return fe_py.Model.lib()
+
+
+def write(tmp_path, text):
+    p = tmp_path / "case.in"
+    p.write_text(text)
+    return str(p)
+
+
+def base_text(**over):
+    params = dict(fe_py.CFG0, Nx=16, Ny=12, Nz=20)
+    params.update(over)
+    return "\n".join(f"{k} = {v} # comment" for k, v in params.items()) + "\n"
+
+
+def setup(tmp_path, text):
+    L = lib()
+    rc = L.fe_setup_host(write(tmp_path, text).encode())
+    return rc
+
+
+def teardown():
+    lib().fe_teardown()
+
+
+def test_exports_match_header(repo):
+    """Every function declared in include/fasteddy_amd.h is exported by the library."""
+    hdr = open(os.path.join(repo, "include", "fasteddy_amd.h")).read()
+    names = re.findall(r"^\s*(?:int|float|double|const char \*)\s*\**\s*(\w+)\s*\(", hdr, re.M)
+    assert len(names) >= 20
+    L = lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_parse_ok_and_unused_report(tmp_path):
+    text = base_text() + "someUnusedKey = 7\n"
+    assert setup(tmp_path, text) == 0
+    L = lib()
+    assert L.getParameterErrors() == 0
+    assert L.countUnusedParameters() >= 1      # someUnusedKey (and Description) are never queried
+    teardown()
+
+
+def test_comments_trim_and_first_equals(tmp_path):
+    text = base_text().replace("Nx = 16 # comment", "   Nx   =   16   # x = 99 inside a comment")
+    assert setup(tmp_path, text) == 0
+    v = C.c_int()
+    lib().fe_get_int(b"Nx", C.byref(v))
+    assert v.value == 16
+    teardown()
+
+
+def test_duplicate_key_is_an_error(tmp_path):
+    rc = setup(tmp_path, base_text() + "Nx = 32\n")
+    assert rc != 0
+    teardown()
+
+
+def test_missing_mandatory_is_an_error(tmp_path):
+    text = "\n".join(l for l in base_text().splitlines() if not l.startswith("hydroBCs")) + "\n"
+    assert setup(tmp_path, text) != 0
+    teardown()
+
+
+def test_out_of_range_is_an_error(tmp_path):
+    assert setup(tmp_path, base_text(thetaAmplitude=3.0)) != 0   # limits [0, 2]
+    teardown()
+    assert setup(tmp_path, base_text(hydroBCs=1)) != 0           # only 2 is valid
+    teardown()
+
+
+def test_optional_defaults(tmp_path):
+    assert setup(tmp_path, base_text()) == 0
+    v = C.c_int()
+    lib().fe_get_int(b"physics_oneRKonly", C.byref(v))
+    assert v.value == 1
+    lib().fe_get_int(b"pgfSelector", C.byref(v))
+    assert v.value == 1
+    teardown()
+
+
+def test_multi_rank_request_is_rejected(tmp_path):
+    assert setup(tmp_path, base_text(numProcsX=2)) != 0
+    teardown()
+
+
+def test_glibc_rand_sequence():
+    """fe_srand/fe_rand reproduce the C library's srand/rand (the reference's IC generator)."""
+    libc = C.CDLL("libc.so.6")
+    L = lib()
+    for seed in (12345, 1, 0, 2024):
+        libc.srand(seed)
+        L.fe_srand(seed)
+        ours = [L.fe_rand() for _ in range(2000)]
+        theirs = [libc.rand() for _ in range(2000)]
+        assert ours == theirs
+
+
+def test_flat_metrics_and_coordinates(tmp_path):
+    p = dict(fe_py.CFG0, Nx=16, Ny=12, Nz=20)
+    m = fe_py.Model(p, str(tmp_path), device=False)
+    assert np.all(m.array("J31") == 0) and np.all(m.array("J32") == 0)
+    assert np.all(m.array("J33") == 1) and np.all(m.array("D_Jac") == 1) and np.all(m.array("invD_Jac") == 1)
+    z = m.array("zPos")[NH, NH, :]
+    np.testing.assert_allclose(z, (np.arange(p["Nz"] + 2 * NH) - NH + 0.5) * 20.0)
+    m.close()
+
+
+def test_stretched_grid_metrics(tmp_path):
+    """verticalDeformSwitch=1: J33 = 1/(dz/dzeta) by centred differences of zDeform."""
+    p = dict(fe_py.CFG0, Nx=8, Ny=8, Nz=30, verticalDeformSwitch=1, verticalDeformFactor=0.6, verticalDeformQuadCoeff=0.0)
+    m = fe_py.Model(p, str(tmp_path), device=False)
+    z = m.array("zPos")[NH, NH, :].astype(np.float64)
+    J33 = m.array("J33")[NH, NH, :]
+    k = np.arange(NH, NH + p["Nz"])
+    dzdzeta = (z[k + 1] - z[k - 1]) / (2 * 20.0)
+    np.testing.assert_allclose(J33[k], 1.0 / dzdzeta, rtol=2e-6)
+    assert np.all(np.diff(z[NH:-NH]) > 0)
+    m.close()
+
+
+def test_base_state_profiles(tmp_path):
+    """Scheme 1: theta = theta_grnd below zStableBottom, + gradient above; p from the
+    analytic hydrostatic Exner profile; rho from the equation of state."""
+    p = dict(fe_py.CFG0, Nx=8, Ny=8, Nz=48)
+    m = fe_py.Model(p, str(tmp_path), device=False)
+    bs = m.array("hydroBaseStateFlds")
+    rho, rth = bs[0][NH, NH, :].astype(np.float64), bs[1][NH, NH, :].astype(np.float64)
+    z = m.array("zPos")[NH, NH, :].astype(np.float64)
+    th = rth / rho
+    Rg, cv = 287.04, 718.0
+    cp = Rg + cv
+    expect_th = np.where(z <= 600.0, 300.0, 300.0 + 0.008 * (z - 600.0))
+    np.testing.assert_allclose(th, expect_th, rtol=2e-6)
+    # hydrostatic Exner: pi(z) = 1 - g/cp int dz/theta
+    g = 9.81
+    ex = np.where(z <= 600, 1 - g / cp * z / 300.0,
+                  1 - g / cp * (600 / 300.0 + np.log(1 + 0.008 * (z - 600) / 300.0) / 0.008))
+    pres = 1e5 * ex ** (cp / Rg)
+    T = th * ex
+    np.testing.assert_allclose(rho, pres / (Rg * T), rtol=2e-5)
+    # discrete hydrostatic balance dp/dz = -rho g to second order
+    k = np.arange(NH + 1, NH + p["Nz"] - 1)
+    dpdz = (pres[k + 1] - pres[k - 1]) / (z[k + 1] - z[k - 1])
+    np.testing.assert_allclose(dpdz, -rho[k] * g, rtol=1e-3)
+    m.close()
+
+
+def test_initial_conditions(tmp_path):
+    """Geostrophic momentum, w = 0, e = 0, theta perturbation within +-A below thetaHeight
+    drawn from srand(12345)/rand() in i,j,k loop order."""
+    p = dict(fe_py.CFG0, Nx=8, Ny=8, Nz=30)
+    m = fe_py.Model(p, str(tmp_path), device=False)
+    h = m.array("hydroFlds")
+    bs = m.array("hydroBaseStateFlds")
+    sl = (slice(NH, -NH),) * 3
+    np.testing.assert_allclose(h[1][sl], 10.0 * bs[0][sl], rtol=1e-7)
+    assert np.all(h[3][sl] == 0) and np.all(h[2][sl] == 0)
+    assert np.all(m.array("sgstkeScalars") == 0)
+    th = (h[4] / h[0])[sl].astype(np.float64) - (bs[1] / bs[0])[sl]
+    z = m.array("zPos")[sl]
+    assert np.abs(th[z <= 400]).max() <= 0.25 + 1e-4
+    assert np.abs(th[z <= 400]).max() > 0.2
+    assert np.all(np.abs(th[z > 400]) < 1e-4)
+    # the perturbation sequence is glibc rand() seeded with 12345, in i, j, k order
+    libc = C.CDLL("libc.so.6")
+    libc.srand(12345)
+    nzp = int((z[0, 0, :] <= 400).sum())
+    first = []
+    for i in range(2):
+        for k in range(nzp):
+            first.append(libc.rand())
+    r0 = np.float32(first[0]) / np.float32(2147483647)
+    expect = (np.float32(bs[1][NH, NH, NH] / bs[0][NH, NH, NH]) + 2.0 * np.float32(0.25) * (float(r0) - 0.5)) * float(h[0][NH, NH, NH])
+    assert h[4][NH, NH, NH] == pytest.approx(np.float32(expect), rel=1e-7)
+    m.close()

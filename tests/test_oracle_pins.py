@@ -1,0 +1,352 @@
+"""Pins of the CPU oracle against closed forms, invariants and textbook results.
+
+The reference ships no tests or golden vectors for its time step, and its
+executable cannot be built here (needs nvcc, MPI and netCDF), so every oracle
+function is pinned by mathematics:
+  RK3-WS2002 amplification factor; advection exactness on polynomials and
+  convergence order; discrete mass conservation; geostrophic/hydrostatic rest
+  gives F == 0; Rayleigh sin^2 profile; neutral surface-layer drag
+  coefficient and Businger-Dyer unstable functions; SGS length-scale branches,
+  pure-shear stress and N^2 of a linear theta; Flatau saturation vapour
+  pressure values; FP32 vs FP64 agreement.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fe_py
+
+NH = 3
+
+
+def small(**kw):
+    p = dict(fe_py.CFG0, Nx=16, Ny=12, Nz=20, thetaPerturbationSwitch=0)
+    p.update(kw)
+    return p
+
+
+def host_model(p, tmp_path):
+    return fe_py.Model(p, str(tmp_path), device=False)
+
+
+def I(a):
+    return a[..., NH:-NH, NH:-NH, NH:-NH]
+
+
+def test_rk3_ws2002_amplification():
+    """y' = lam*y: one RK3-WS2002 step multiplies by 1 + z + z^2/2 + z^3/6 (cuda_RKschemes.cu:113-136)."""
+    p = small(dt=0.1)
+    o = fe_py.Oracle(p, 64)
+    lam = -2.3
+    y0 = np.random.default_rng(0).uniform(0.5, 1.5, size=o.arr("HYDRO").shape)
+    h = o.arr("HYDRO")
+    h[...] = y0
+    for st in range(3):
+        o.arr("HYDRO_F")[...] = lam * h
+        o.rk_stage(st)
+    z = lam * float(np.float32(0.1))
+    amp = 1 + z + z * z / 2 + z ** 3 / 6
+    np.testing.assert_allclose(I(h), I(y0) * amp, rtol=1e-14)
+    o.close()
+
+
+@pytest.mark.parametrize("bhyb", [0.0, 1.0])
+def test_hyb56_polynomial_exactness(bhyb):
+    """5th/6th-order flux-form advection is exact for degree <= 4 data in a uniform flow."""
+    p = small(Nx=40, Ny=8, Nz=8, b_hyb=bhyb, pgfSelector=0, buoyancySelector=0, coriolisSelector=0,
+              turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0, d_xi=20.0)
+    nx = p["Nx"]
+    dx = p["d_xi"]
+    x = (np.arange(nx + 2 * NH) - NH + 0.5) * dx
+    c = [0.3, -0.02, 1.5e-3, -2e-5, 1e-7]
+    poly = sum(ci * (x / 100.0) ** n for n, ci in enumerate(c)) + 300.0
+    dpoly = sum(n * ci * (x / 100.0) ** (n - 1) / 100.0 for n, ci in enumerate(c) if n > 0)
+    U = 3.0
+    o = fe_py.Oracle(p, 64)
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[1] = U
+    h[2] = 0.0
+    h[3] = 0.0
+    h[4] = poly[:, None, None]
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = 300.0
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    o.build_frhs(0)
+    F = o.arr("HYDRO_F")[4]
+    # cells at least 4 away from the periodic seam see only the polynomial
+    sel = slice(NH + 4, NH + nx - 4)
+    np.testing.assert_allclose(F[sel, NH:-NH, NH:-NH], (-U * dpoly[sel])[:, None, None] * np.ones_like(F[sel, NH:-NH, NH:-NH]),
+                               rtol=0, atol=2e-11 * 300 * U / dx)
+    o.close()
+
+
+def _sine_error(n, scheme, bhyb):
+    L = 1000.0
+    dx = L / n
+    p = small(Nx=n, Ny=4, Nz=4, d_xi=dx, advectionSelector=scheme, b_hyb=bhyb, pgfSelector=0, buoyancySelector=0,
+              coriolisSelector=0, turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
+    x = (np.arange(n + 2 * NH) - NH + 0.5) * dx
+    U = 2.0
+    o = fe_py.Oracle(p, 64)
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[1] = U
+    h[2] = 0.0
+    h[3] = 0.0
+    h[4] = (300.0 + np.sin(2 * np.pi * x / L))[:, None, None]
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = 300.0
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    o.build_frhs(0)
+    F = I(o.arr("HYDRO_F"))[4][:, 0, 0].copy()
+    xe = x[NH:-NH]
+    exact = -U * (2 * np.pi / L) * np.cos(2 * np.pi * xe / L)
+    o.close()
+    return np.abs(F - exact).max()
+
+
+@pytest.mark.parametrize("scheme,bhyb,order", [(3, 0.0, 5), (3, 1.0, 6), (2, 0.0, 3), (2, 1.0, 4), (6, 0.0, 2)])
+def test_advection_convergence_order(scheme, bhyb, order):
+    e1, e2 = _sine_error(32, scheme, bhyb), _sine_error(64, scheme, bhyb)
+    rate = math.log2(e1 / e2)
+    assert rate > order - 0.35, (rate, e1, e2)
+
+
+def test_constant_fields_have_no_advective_tendency():
+    for scheme in (0, 1, 2, 3, 4, 5, 6):
+        p = small(advectionSelector=scheme, pgfSelector=0, buoyancySelector=0, coriolisSelector=0,
+                  turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
+        o = fe_py.Oracle(p, 64)
+        h = o.arr("HYDRO")
+        rng = np.random.default_rng(scheme)
+        h[0] = 1.1
+        h[1] = 1.1 * rng.uniform(-3, 3, size=h[1].shape)
+        h[2] = 1.1 * rng.uniform(-3, 3, size=h[1].shape)
+        h[3] = 1.1 * rng.uniform(-1, 1, size=h[1].shape)
+        h[4] = 330.0
+        o.arr("BS")[0] = 1.1
+        o.arr("BS")[1] = 330.0
+        for a in ("J33", "DJAC", "INVDJAC"):
+            o.arr(a)[...] = 1.0
+        o.build_frhs(0)
+        # a constant field advected by any (non-divergence-free) flow: F = -phi * div(u) exactly, so
+        # for phi == const the tendency of rho*theta/rho*... equals const * F_rho
+        F = I(o.arr("HYDRO_F"))
+        np.testing.assert_allclose(F[4], (330.0 / 1.1) * F[0], rtol=1e-9, atol=1e-9)
+        o.close()
+
+
+def test_mass_conservation(tmp_path):
+    """Sum of F_rho over the periodic domain vanishes (flux form, w_cf = 0 at ground and lid)."""
+    p = small()
+    m = host_model(p, tmp_path)
+    fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=5)
+    o = fe_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    for st in (0, 2):
+        o.build_frhs(st)
+        Fr = I(o.arr("HYDRO_F"))[0]
+        assert abs(Fr.sum()) <= 1e-12 * np.abs(Fr).sum()
+    o.close()
+
+
+def test_geostrophic_rest_has_zero_tendency(tmp_path):
+    """rho=rho_BS, rho*theta=(rho*theta)_BS, u=U_g, w=0, e=0: stage-1 F is identically zero."""
+    p = small(surflayerSelector=0)
+    m = host_model(p, tmp_path)
+    o = fe_py.Oracle(p, 32)
+    o.load_from_model(m)
+    m.close()
+    o.build_frhs(0)
+    assert np.all(I(o.arr("HYDRO_F")) == 0.0)
+    assert np.all(I(o.arr("TKE_F")) == 0.0)
+    o.close()
+
+
+def test_rayleigh_profile(tmp_path):
+    """Only Rayleigh damping acts on a uniform w: F_w = -0.2 rho sin^2(pi/2 (1-(z_top-z)/z_d)) w/rho."""
+    p = small(surflayerSelector=0, coriolisSelector=0, buoyancySelector=0, pgfSelector=0, turbulenceSelector=0,
+              TKESelector=0, dampingLayerDepth=150.0, advectionSelector=3)
+    m = host_model(p, tmp_path)
+    o = fe_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[1:4] = 0.0
+    h[3] = 0.5
+    h[4] = 300.0
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = 300.0
+    o.build_frhs(0)
+    z = o.arr("ZPOS")[0]
+    ztop = z[NH, NH, -NH - 1]
+    Fw = o.arr("HYDRO_F")[3]
+    # advection of a uniform w: no divergence where the 6-point stencil stays clear of the ground/lid halos
+    for k in range(NH + 3, NH + p["Nz"] - 3):
+        zz = z[NH, NH, k]
+        expect = 0.0
+        if zz >= ztop - 150.0:
+            expect = -0.2 * math.sin(math.pi / 2 * (1 - (ztop - zz) / 150.0)) ** 2 * 0.5
+        assert Fw[NH + 2, NH + 2, k] == pytest.approx(expect, abs=1e-12)
+    o.close()
+
+
+def _sfc_oracle(tmp_path, wth, stab, U=6.0):
+    p = small(surflayerSelector=1, surflayer_wth=wth, surflayer_stab=stab, U_g=U, coriolisSelector=0)
+    m = host_model(p, tmp_path)
+    o = fe_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    o.build_frhs(2)
+    return o, p
+
+
+def test_surface_layer_neutral_drag(tmp_path):
+    """surflayer_stab = 1: 1/L = 0 and c_d = kappa^2 / ln^2((z1+z0)/z0)."""
+    o, p = _sfc_oracle(tmp_path, 0.35, 1)
+    f32 = lambda v: float(np.float32(v))   # .in values and kappa are float32 in the reference
+    z1, z0, kap = 0.5 * p["d_zeta"], f32(0.05), f32(0.4)
+    cd = kap ** 2 / math.log((z1 + z0) / z0) ** 2
+    cdf = o.arr("CD")[NH:-NH, NH:-NH]
+    np.testing.assert_allclose(cdf, cd, rtol=1e-12)
+    # tau_13 = -c_d |U| rho u at k = kMin
+    h = o.arr("HYDRO")
+    rho, ru = h[0][NH:-NH, NH:-NH, NH], h[1][NH:-NH, NH:-NH, NH]
+    t13 = o.arr("TAU")[2][NH:-NH, NH:-NH, NH]
+    np.testing.assert_allclose(t13, -cd * np.abs(ru / rho) * ru, rtol=1e-12)
+    np.testing.assert_allclose(o.arr("TAU")[8][NH:-NH, NH:-NH, NH], f32(0.35) * rho, rtol=1e-12)
+    o.close()
+
+
+def test_surface_layer_unstable_businger_dyer(tmp_path):
+    """Unstable branch against an independent Businger-Dyer (Paulson 1970) evaluation."""
+    o, p = _sfc_oracle(tmp_path, 0.35, 0, U=6.0)
+    f32 = lambda v: float(np.float32(v))
+    z1, z0, kap, g = 0.5 * p["d_zeta"], f32(0.05), f32(0.4), f32(9.81)
+    h = o.arr("HYDRO")
+    i, j = NH + 1, NH + 1
+    rho, u, th = h[0][i, j, NH], h[1][i, j, NH] / h[0][i, j, NH], h[4][i, j, NH] / h[0][i, j, NH]
+    U1 = abs(u)
+    cd = kap ** 2 / math.log((z1 + z0) / z0) ** 2
+    for _ in range(6):
+        ust = math.sqrt(cd) * U1
+        invL = max(min(-(kap * g * f32(0.35)) / (ust ** 3 * th), 1.0), -1.0)
+        zeta = invL * (z1 + z0)
+        x = (1 - 16 * zeta) ** 0.25
+        psi_m = 2 * math.log((1 + x) / 2) + math.log((1 + x * x) / 2) - 2 * math.atan(x) + math.pi / 2
+        cd = kap ** 2 / (math.log((z1 + z0) / z0) - psi_m) ** 2
+    assert o.arr("CD")[i, j] == pytest.approx(cd, rel=1e-10)
+    o.close()
+
+
+def test_sgs_length_scale_branches_and_shear_stress():
+    """e = 0 in a stable column gives l = 0.01 m; a neutral column gives l = Delta; pure shear
+    u = alpha z with uniform e gives tau_13 = -2 rho Km alpha/2 with Km = c_k Delta sqrt(e)."""
+    p = small(Nx=8, Ny=8, Nz=12, surflayerSelector=0, coriolisSelector=0, d_xi=10.0, d_eta=10.0, d_zeta=10.0)
+    o = fe_py.Oracle(p, 64)
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    nz = p["Nz"] + 2 * NH
+    z = (np.arange(nz) - NH + 0.5) * 10.0
+    o.arr("ZPOS")[0] = z[None, None, :]
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[2] = 0.0
+    h[3] = 0.0
+    alpha = 0.02
+    h[1] = alpha * z[None, None, :]
+    # stable: theta increasing with z; e = 0
+    h[4] = 300.0 + 0.01 * z[None, None, :]
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = h[4]
+    o.arr("TKE")[0] = 0.0
+    o.build_frhs(2)
+    f32 = lambda v: float(np.float32(v))
+    ls = o.arr("LS")[0][NH:-NH, NH:-NH, NH + 1:-NH - 1]
+    np.testing.assert_allclose(ls, f32(0.01), rtol=1e-12)
+    # neutral column, uniform e
+    h[4] = 300.0
+    o.arr("BS")[1] = 300.0
+    e0 = 0.3
+    o.arr("TKE")[0] = e0
+    o.build_frhs(2)
+    delta = 10.0
+    ls = o.arr("LS")[0][NH:-NH, NH:-NH, NH + 1:-NH - 1]
+    np.testing.assert_allclose(ls, delta, rtol=1e-12)
+    km = f32(0.10) * delta * math.sqrt(e0)
+    t31 = o.arr("TAU")[2][NH:-NH, NH:-NH, NH + 2:-NH - 2]
+    np.testing.assert_allclose(t31, -2 * km * alpha / 2, rtol=1e-10)
+    o.close()
+
+
+def test_brunt_vaisala_of_linear_theta():
+    """theta = theta0 + G z: N^2 = g G / theta (length scale 0.76 sqrt(e)/N when below Delta)."""
+    p = small(Nx=8, Ny=8, Nz=12, surflayerSelector=0, coriolisSelector=0, d_xi=50.0, d_eta=50.0, d_zeta=50.0)
+    o = fe_py.Oracle(p, 64)
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    nz = p["Nz"] + 2 * NH
+    z = (np.arange(nz) - NH + 0.5) * 50.0
+    o.arr("ZPOS")[0] = z[None, None, :]
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[1:4] = 0.0
+    G = 0.05
+    th = 300.0 + G * z[None, None, :]
+    h[4] = th
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = h[4]
+    e0 = 0.04
+    o.arr("TKE")[0] = e0
+    o.build_frhs(2)
+    k = NH + 5
+    N2 = float(np.float32(9.81)) * G / th[0, 0, k]
+    expect = 0.76 * math.sqrt(e0) / math.sqrt(N2)
+    assert o.arr("LS")[0][NH + 1, NH + 1, k] == pytest.approx(expect, rel=1e-10)
+    o.close()
+
+
+def test_flatau_saturation_vapour_pressure(tmp_path):
+    """The liquid-water polynomial reproduces tabulated e_s: 6.112 hPa at 0 C, 23.39 hPa at 20 C,
+    42.46 hPa at 30 C (WMO/Flatau 1992), and a sub-saturated cloud-free cell has fcond = 0."""
+    a = [6.11239921, 0.443987641, 0.142986287e-1, 0.264847430e-3, 0.302950461e-5, 0.206739458e-7,
+         0.640689451e-10, -0.952447341e-13, -0.976195544e-15]
+    es = lambda tc: sum(c * tc ** n for n, c in enumerate(a))
+    assert es(0.0) == pytest.approx(6.112, rel=2e-3)
+    assert es(20.0) == pytest.approx(23.39, rel=2e-3)
+    assert es(30.0) == pytest.approx(42.46, rel=2e-3)
+    p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=10, thetaPerturbationSwitch=0, surflayerSelector=0)
+    m = host_model(p, tmp_path)
+    o = fe_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    mq = o.arr("MOIST")
+    mq[0] = 5.0 * o.arr("HYDRO")[0]      # qv = 5 g/kg: sub-saturated near 300 K
+    mq[1] = 0.0
+    o.build_frhs(2)
+    assert np.all(I(o.arr("FCOND")) == 0.0)
+    o.close()
+
+
+def test_fp32_oracle_tracks_fp64(tmp_path):
+    p = small()
+    m = host_model(p, tmp_path)
+    fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=11)
+    o32, o64 = fe_py.Oracle(p, 32), fe_py.Oracle(p, 64)
+    o32.load_from_model(m)
+    o64.load_from_model(m)
+    m.close()
+    for st in (0, 2):
+        o32.build_frhs(st)
+        o64.build_frhs(st)
+        a, b = I(o32.arr("HYDRO_F")).astype(np.float64), I(o64.arr("HYDRO_F"))
+        rel = np.abs(a - b).max(axis=(1, 2, 3)) / np.abs(b).max(axis=(1, 2, 3))
+        assert np.all(rel < 2e-4), rel
+    o32.close()
+    o64.close()

@@ -11,8 +11,8 @@
  * One RK3 step is three fused stage launches with out-of-place buffers:
  *   stage 1: S0 -> S1, stage 2: (S1, S0) -> S2, stage 3: (S2, S0) -> S0,
  * the physics stage preceded by a column pass (SGS length scale, eddy
- * viscosity, surface layer).  A step is captured once in a hipGraph and
- * replayed.
+ * viscosity, surface layer).  All launches go to one non-blocking stream;
+ * the host synchronises once per batch.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>

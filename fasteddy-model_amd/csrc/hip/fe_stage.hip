@@ -27,11 +27,12 @@
 
 namespace fe {
 
-constexpr int TX = 64, TY = 4, NT = TX * TY, HH = 3;
-constexpr int RW = TX + 2 * HH, RH = TY + 2 * HH;      /* raw plane with 3-cell halo        */
-constexpr int DW = TX + 2, DH = TY + 2;                /* derived region [-1,TX] x [-1,TY]  */
-constexpr int NHALO = RW * RH - TX * TY;
-constexpr int NGX = (TX + 1) * TY, NGY = TX * (TY + 1);
+constexpr int TX = 64, TY = 8, NT = TX * TY, HH = 3;
+constexpr int RW = TX + 2 * HH, RH = TY + 2 * HH;      /* raw plane with 3-cell halo          */
+constexpr int DW = TX + 2, DH = TY + 2;                /* ring-1 region [-1,TX] x [-1,TY]     */
+constexpr int NHALO = RW * RH - TX * TY;               /* halo cells of the raw plane (<= NT) */
+constexpr int NR1 = DW * DH - TX * TY;                 /* ring-1 cells, listed first          */
+static_assert(NHALO <= NT, "one halo cell per thread");
 
 __device__ __forceinline__ int wrapi(int v, int n) {
   v %= n;
@@ -314,261 +315,359 @@ __global__ void __launch_bounds__(256) column_kernel(StepArgs a, int do_sgs, int
 }
 
 /* =============================================================================== */
+/* Halo cell h of a tile, in tile-local coordinates: the ring-1 cells (the ring
+ * around the tile that the derived planes need) come first, then the rest of
+ * the 3-cell halo of the raw plane. */
+__device__ __forceinline__ void halo_cell(int h, int &lx, int &ly) {
+  if (h < NR1) {
+    if (h < DW) { ly = -1; lx = h - 1; }
+    else if (h < 2 * DW) { ly = TY; lx = h - DW - 1; }
+    else if (h < 2 * DW + TY) { lx = -1; ly = h - 2 * DW; }
+    else { lx = TX; ly = h - 2 * DW - TY; }
+  } else {
+    int q = h - NR1;
+    if (q < 2 * RW) { ly = -3 + q / RW; lx = -3 + q % RW; }
+    else if (q < 4 * RW) { q -= 2 * RW; ly = TY + 1 + q / RW; lx = -3 + q % RW; }
+    else { q -= 4 * RW; ly = -1 + q / 4; int c = q % 4; lx = c < 2 ? -3 + c : TX + 1 + (c - 2); }
+  }
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+/* One fused RK stage.  Per level z, with two work-group barriers:
+ *   P1  write the xy plane of level z (own cell from the register window,
+ *       halo from registers prefetched one or two levels earlier), rho^-1 and
+ *       p' on the ring-1 region; physics stage: velocities of level z+1 and
+ *       rho*Km terms of level z on the ring-1 region;
+ *   P2  x and y face fluxes (each face once), SGS stresses on x and y faces,
+ *       cell-centred TKE transport fluxes;
+ *   P3  own-cell tendency (flux differences, z faces from registers, pressure
+ *       gradient, buoyancy, damping, Coriolis, filter, SGS divergence, TKE
+ *       sources, microphysics) and the RK update.
+ * The p' plane is consumed in P2 (own-cell horizontal differences go to
+ * registers), so one copy suffices; velocity planes read in P3 and written in
+ * the next P1 rotate over four levels (u, v, w: levels z-1..z+1 are live) or
+ * three (theta and moisture: z, z+1). */
 template <class C>
-__global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS ? 2 : 4, 8))) stage_kernel(StepArgs a) {
   constexpr int NF = C::NF;
+  constexpr bool PH = C::PHYS;
+  constexpr int PD = PH ? 2 : 1;                 /* halo prefetch distance (levels) */
+  constexpr int NV = PH ? C::NV : 1;
+  constexpr int NB = PH ? C::NB : 1;
   __shared__ float raw[NF][RH][RW];
   __shared__ float dvr[DH][DW];
   __shared__ float dvp[DH][DW];
   __shared__ float gx[NF][TY][TX + 1];
   __shared__ float gy[NF][TY + 1][TX];
-  /* physics stage only */
-  constexpr int NVB = C::PHYS ? 3 : 1;
-  constexpr int NVv = C::PHYS ? C::NV : 1;
-  __shared__ float vel[NVB][NVv][DH][DW];
-  __shared__ float kmp[C::PHYS ? 3 : 1][C::PHYS ? DH : 1][C::PHYS ? DW : 1];
-  __shared__ float tb1[C::PHYS ? C::NB : 1][C::PHYS ? TY : 1][C::PHYS ? TX + 1 : 1];
-  __shared__ float tb2[C::PHYS ? C::NB : 1][C::PHYS ? TY + 1 : 1][C::PHYS ? TX : 1];
+  /* velocities u,v,w rotate over four levels, theta and moisture over three */
+  __shared__ float velm[PH ? 4 : 1][PH ? 3 : 1][DH][DW];
+  __shared__ float vels[PH ? 3 : 1][PH ? NV - 3 : 1][DH][DW];
+  __shared__ float kmp[PH ? 2 : 1][PH ? DH : 1][PH ? DW : 1];
+  __shared__ float tb1[NB][PH ? TY : 1][PH ? TX + 1 : 1];
+  __shared__ float tb2[NB][PH ? TY + 1 : 1][PH ? TX : 1];
+  __shared__ float qx[PH ? TY : 1][PH ? TX + 2 : 1];
+  __shared__ float qy[PH ? TY + 2 : 1][PH ? TX : 1];
+
+  typedef float Plane[DH][DW];
+  auto VL = [&](int lev, int q) -> Plane & { return q < 3 ? velm[lev & 3][q] : vels[(lev + 3) % 3][q - 3]; };
 
   const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX;
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int tx = tid % TX, ty = tid / TX, wave = ty;
+  /* XCD-aware tile order: hardware round-robins work-groups over the 8 XCDs, so
+   * give each XCD a contiguous run of tiles (neighbours then share an L2). */
+  const int nbx = gridDim.x, nbt = gridDim.x * gridDim.y;
+  int b = blockIdx.x + gridDim.x * blockIdx.y;
+  if ((nbt & 7) == 0) b = (b & 7) * (nbt >> 3) + (b >> 3);
+  const int x0 = (b % nbx) * TX, y0 = (b / nbx) * TY;
   const bool active = (x0 + tx < a.Nx) && (y0 + ty < a.Ny);
   const int cx = wrapi(x0 + tx, a.Nx), cy = wrapi(y0 + ty, a.Ny);
-  const long col = (long)cy * a.P + cx;
+  const int col = cy * a.P + cx;
   const int Nz = a.Nz;
-
-  /* horizontal halo cells of this tile: LDS slot and global column, fixed for all levels */
-  int hslot[2];
-  long hcol[2];
+  const int ps = (int)a.ps;
+  const float *fin[NF];
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
-    int h = tid + p * NT;
-    hslot[p] = -1;
-    hcol[p] = 0;
-    if (h < NHALO) {
-      int hx, hy;
-      if (h < HH * RW) { hy = h / RW; hx = h % RW; }
-      else if (h < 2 * HH * RW) { int hh = h - HH * RW; hy = TY + HH + hh / RW; hx = hh % RW; }
-      else { int hh = h - 2 * HH * RW; hy = HH + hh / (2 * HH); int c = hh % (2 * HH); hx = c < HH ? c : TX + c; }
-      hslot[p] = hy * RW + hx;
-      hcol[p] = (long)wrapi(y0 + hy - HH, a.Ny) * a.P + wrapi(x0 + hx - HH, a.Nx);
-    }
+  for (int f = 0; f < NF; ++f) fin[f] = a.in + (long)f * a.fs;
+
+  /* this thread's halo cell */
+  int hlx = 0, hly = 0, hcol = 0, hslot = -1;
+  const bool hasH = tid < NHALO;
+  const bool hasR1 = tid < NR1;
+  if (hasH) {
+    halo_cell(tid, hlx, hly);
+    hslot = (hly + HH) * RW + (hlx + HH);
+    hcol = wrapi(y0 + hly, a.Ny) * a.P + wrapi(x0 + hlx, a.Nx);
   }
+  const int hd = (hly + 1) * DW + (hlx + 1);           /* ring-1 slot (valid when hasR1) */
+  const int od = (ty + 1) * DW + (tx + 1);             /* own slot in the ring-1 region  */
+
+  auto cv = [&](int f, int z, int c) -> float {       /* value with vertical boundary rules */
+    if (z >= 0 && z < Nz) return fin[f][z * ps + c];
+    if (f == 0) return prof(a, PR_RHO, z);
+    if (f == 4) return prof(a, PR_RTH, z);
+    if (z < 0) return fin[f][c];
+    if (f == 1) return a.uTop;
+    if (f == 2) return a.vTop;
+    if (f == 3) return 0.f;
+    if (C::TKE && f == C::IE) return 0.f;
+    return fin[f][(Nz - 1) * ps + c];
+  };
+  auto rthm_of = [&](float rth, float rho, float qv) {
+    if constexpr (C::NM > 0) return rth * (1.f + a.Rv_Rg * qv * frcp(rho) * 1e-3f);
+    else return rth;
+  };
 
   float omb[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     int s = (f < 5) ? C::SH : ((C::TKE && f == C::IE) ? C::SE : (f == C::IQV ? C::SQV : C::SQL));
-    float b = (f < 5) ? a.bh : ((C::TKE && f == C::IE) ? a.be : a.bqv);
-    omb[f] = (s == 3 || s == 2) ? b : 0.f;
+    float bb = (f < 5) ? a.bh : ((C::TKE && f == C::IE) ? a.be : a.bqv);
+    omb[f] = (s == 3 || s == 2) ? bb : 0.f;
   }
 
-  /* own-column register window: win[f][j] = phi_f(z - 3 + j) */
+  /* register window of the own column, z-3..z+3 */
   float win[NF][7];
 #pragma unroll
   for (int f = 0; f < NF; ++f)
 #pragma unroll
-    for (int j = 0; j < 7; ++j) win[f][j] = colval<C>(a, f, j - 3, col);
+    for (int j = 0; j < 7; ++j) win[f][j] = cv(f, j - 3, col);
+  /* halo values of levels z .. z+PD */
+  float hv[PD + 1][NF];
+#pragma unroll
+  for (int l = 0; l < PD; ++l)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) hv[l][f] = hasH ? cv(f, l, hcol) : 0.f;
 
-  auto rthm_of = [&](float rth, float rho, float qv) {
-    if constexpr (C::NM > 0) return rth * (1.f + a.Rv_Rg * qv / rho * 1e-3f);
-    else return rth;
-  };
-  /* own-column perturbation pressure at z-1, z, z+1 (halo levels take the kMin / kMax-1 value) */
   float pz[3];
   {
-    float qv3 = C::NM > 0 ? win[C::IQV][3] : 0.f, qv4 = C::NM > 0 ? win[C::IQV][4] : 0.f;
-    pz[1] = pprime(a, rthm_of(win[4][3], win[0][3], qv3), 0);
+    float q3 = C::NM > 0 ? win[C::IQV][3] : 0.f, q4 = C::NM > 0 ? win[C::IQV][4] : 0.f;
+    pz[1] = pprime(a, rthm_of(win[4][3], win[0][3], q3), 0);
     pz[0] = pz[1];
-    pz[2] = (Nz > 1) ? pprime(a, rthm_of(win[4][4], win[0][4], qv4), 1) : pz[1];
+    pz[2] = (Nz > 1) ? pprime(a, rthm_of(win[4][4], win[0][4], q4), 1) : pz[1];
   }
   float gzlo[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) gzlo[f] = 0.f;   /* w_cf = 0 at the ground */
-  float rin = 1.f / win[0][3];                 /* rho^-1 at own z */
+  for (int f = 0; f < NF; ++f) gzlo[f] = 0.f;
+  float rin = frcp(win[0][3]);
 
   /* ---- physics-stage carried state ---- */
-  float zc[C::PHYS ? 4 + C::NM : 1];           /* own-column z-face stresses at z: T31, T32, T33, TH3, TQ3... */
-  float Aown[3], riz[5];                       /* rho*Km at z-1..z+1; rho^-1 at z-2..z+2 */
-  int vb0 = 0, vb1 = 1, vb2 = 2;               /* vel buffers holding z-1, z, z+1 */
-  if constexpr (C::PHYS) {
+  float zc[PH ? 4 + C::NM : 1];
+  float Aown[3] = {0.f, 0.f, 0.f};     /* rho*Km own column at z-1, z, z+1 */
+  float riz[5];                         /* rho^-1 own column at z-2..z+2    */
+  float kmo = 0.f, lso = 0.f;           /* own Km, l at level z             */
+  float kmo1 = 0.f, lso1 = 0.f;         /* own Km, l at level z+1           */
+  float kh = 0.f, lh = 0.f;             /* ring-1 halo Km, l at level z     */
+  if constexpr (PH) {
 #pragma unroll
-    for (int j = 0; j < 5; ++j) riz[j] = 1.f / win[0][j + 1];
-    /* vel planes at z=-1 and z=0 for the tile region [-1,TX]x[-1,TY] */
-    for (int lev = -1; lev <= 0; ++lev) {
-      int b = lev < 0 ? vb0 : vb1;
-      for (int i = tid; i < DW * DH; i += NT) {
-        int dx = i % DW - 1, dy = i / DW - 1;
-        long c = (long)wrapi(y0 + dy, a.Ny) * a.P + wrapi(x0 + dx, a.Nx);
-        float r = 1.f / colval<C>(a, 0, lev, c);
-        vel[b][0][dy + 1][dx + 1] = colval<C>(a, 1, lev, c) * r;
-        vel[b][1][dy + 1][dx + 1] = colval<C>(a, 2, lev, c) * r;
-        vel[b][2][dy + 1][dx + 1] = colval<C>(a, 3, lev, c) * r;
-        vel[b][3][dy + 1][dx + 1] = colval<C>(a, 4, lev, c) * r;
+    for (int j = 0; j < 5; ++j) riz[j] = frcp(win[0][j + 1]);
+    kmo = a.km[1 * ps + col]; lso = a.ls[1 * ps + col];
+    kmo1 = a.km[2 * ps + col]; lso1 = a.ls[2 * ps + col];
+    if (hasR1) { kh = a.km[1 * ps + hcol]; lh = a.ls[1 * ps + hcol]; }
+    /* u,v,w of levels -1 and 0, theta and moisture of level 0, on the ring-1 region */
+    for (int i = tid; i < DW * DH; i += NT) {
+      int dx = i % DW - 1, dy = i / DW - 1;
+      int c = wrapi(y0 + dy, a.Ny) * a.P + wrapi(x0 + dx, a.Nx);
 #pragma unroll
-        for (int m = 0; m < C::NM; ++m) vel[b][4 + m][dy + 1][dx + 1] = colval<C>(a, C::IQV + m, lev, c) * r;
-      }
-    }
-    /* Km at z-1 (= 0) and z = 0 for the own column */
-    Aown[0] = 0.f;
-    Aown[1] = a.km[1 * a.ps + col] * win[0][3];
-    __syncthreads();
-    /* z-face stresses at z = 0: surface values for T31, T32, TH3 (and TQ3 of qv) */
-    {
-      const long c2 = col;
-      float ls0 = a.ls[1 * a.ps + col];
-      float A = Aown[1];
-      float Bv = A * (1.f + 2.f * ls0 / prof(a, PR_DELTA, 0));
-      float E = C::TKE ? (2.f / 3.f) * win[C::IE][3] : 0.f;
-      const int o = 1 + 1 * DW;   /* own cell offset is (ty+1, tx+1) */
-      (void)o;
-      float w0 = vel[vb1][2][ty + 1][tx + 1], wm = vel[vb0][2][ty + 1][tx + 1];
-      float s33 = prof(a, PR_J33, 0) * a.dZi * (w0 - wm);
-      float s11 = a.dXi * (vel[vb1][0][ty + 1][tx + 1] - vel[vb1][0][ty + 1][tx]);
-      float s22 = a.dYi * (vel[vb1][1][ty + 1][tx + 1] - vel[vb1][1][ty][tx + 1]);
-      zc[0] = a.s.t13[c2];
-      zc[1] = a.s.t23[c2];
-      zc[2] = -2.f * A * s33 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
-      zc[3] = a.s.th3[c2];
+      for (int lev = -1; lev <= 0; ++lev) {
+        float r = frcp(cv(0, lev, c));
+        VL(lev, 0)[dy + 1][dx + 1] = cv(1, lev, c) * r;
+        VL(lev, 1)[dy + 1][dx + 1] = cv(2, lev, c) * r;
+        VL(lev, 2)[dy + 1][dx + 1] = cv(3, lev, c) * r;
+        if (lev == 0) {
+          VL(lev, 3)[dy + 1][dx + 1] = cv(4, lev, c) * r;
 #pragma unroll
-      for (int m = 0; m < C::NM; ++m) {
-        if (m == 0) zc[4] = a.s.tq3[c2];
-        else {
-          float q0 = vel[vb1][4 + m][ty + 1][tx + 1], qm = vel[vb0][4 + m][ty + 1][tx + 1];
-          zc[4 + m] = -Bv * prof(a, PR_J33, 0) * a.dZi * (q0 - qm);
+          for (int m = 0; m < C::NM; ++m) VL(lev, 4 + m)[dy + 1][dx + 1] = cv(C::IQV + m, lev, c) * r;
         }
       }
+    }
+    Aown[0] = 0.f;
+    Aown[1] = kmo * win[0][3];
+    __syncthreads();
+    {
+      auto v0 = [&](int q) -> Plane & { return VL(0, q); };
+      const int Y = ty + 1, X = tx + 1;
+      float A = Aown[1];
+      float Bv = A * (1.f + 2.f * lso / prof(a, PR_DELTA, 0));
+      float E = C::TKE ? (2.f / 3.f) * win[C::IE][3] : 0.f;
+      float s33 = prof(a, PR_J33, 0) * a.dZi * (win[3][3] * riz[2] - win[3][2] * riz[1]);
+      float s11 = a.dXi * (v0(0)[Y][X] - v0(0)[Y][X - 1]);
+      float s22 = a.dYi * (v0(1)[Y][X] - v0(1)[Y - 1][X]);
+      zc[0] = a.s.t13[col];
+      zc[1] = a.s.t23[col];
+      zc[2] = -2.f * A * s33 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+      zc[3] = a.s.th3[col];
+#pragma unroll
+      for (int m = 0; m < C::NM; ++m)
+        zc[4 + m] = (m == 0) ? a.s.tq3[col]
+                             : -Bv * prof(a, PR_J33, 0) * a.dZi * (win[C::IQV + m][3] * riz[2] - win[C::IQV + m][2] * riz[1]);
     }
   }
 
   for (int z = 0; z < Nz; ++z) {
-    /* prefetch the window's next level and the RK base */
+    /* ---- prefetch: own column z+4, halo z+PD, RK base z; physics: Km/l ---- */
     float nxt[NF], bse[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) nxt[f] = colval<C>(a, f, z + 4, col);
+    for (int f = 0; f < NF; ++f) nxt[f] = cv(f, z + 4, col);
+    if (hasH) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) hv[PD][f] = cv(f, z + PD, hcol);
+    }
     if (!a.emitF) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bse[f] = a.base[f * a.fs + (long)z * a.ps + col];
+      for (int f = 0; f < NF; ++f) bse[f] = a.base[(long)f * a.fs + z * ps + col];
     }
-    float kmn = 0.f, lsn = 0.f, lsz = 0.f;
-    if constexpr (C::PHYS) {
-      kmn = a.km[(long)(z + 2) * a.ps + col];
-      lsn = a.ls[(long)(z + 2) * a.ps + col];
-      lsz = a.ls[(long)(z + 1) * a.ps + col];
+    float kmo2 = 0.f, lso2 = 0.f, khn = 0.f, lhn = 0.f;
+    if constexpr (PH) {
+      kmo2 = a.km[(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * ps + col];
+      lso2 = a.ls[(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * ps + col];
+      if (hasR1) { khn = a.km[(z + 2) * ps + hcol]; lhn = a.ls[(z + 2) * ps + hcol]; }
     }
 
-    /* (1) xy plane at level z: own cell from the window, halo ring from memory */
+    /* ---- P1 ---- */
 #pragma unroll
     for (int f = 0; f < NF; ++f) raw[f][ty + HH][tx + HH] = win[f][3];
-    const long zoff = (long)z * a.ps;
+    if (hasH) {
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
-      if (hslot[p] >= 0) {
+      for (int f = 0; f < NF; ++f) (&raw[f][0][0])[hslot] = hv[0][f];
+    }
+    (&dvr[0][0])[od] = rin;
+    (&dvp[0][0])[od] = pz[1];
+    if (hasR1) {
+      float q = C::NM > 0 ? hv[0][C::IQV] : 0.f;
+      (&dvr[0][0])[hd] = frcp(hv[0][0]);
+      (&dvp[0][0])[hd] = pprime(a, rthm_of(hv[0][4], hv[0][0], q), z);
+    }
+    if constexpr (PH) {
+      auto vb = [&](int q) -> Plane & { return VL(z + 1, q); };
+      {
+        float r = frcp(win[0][4]);
+        (&vb(0)[0][0])[od] = win[1][4] * r;
+        (&vb(1)[0][0])[od] = win[2][4] * r;
+        (&vb(2)[0][0])[od] = win[3][4] * r;
+        (&vb(3)[0][0])[od] = win[4][4] * r;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) (&raw[f][0][0])[hslot[p]] = a.in[f * a.fs + zoff + hcol[p]];
+        for (int m = 0; m < C::NM; ++m) (&vb(4 + m)[0][0])[od] = win[C::IQV + m][4] * r;
       }
-    if constexpr (C::PHYS) {
-      /* velocities at z+1 for the tile region (top halo level uses the boundary values) */
-      for (int i = tid; i < DW * DH; i += NT) {
-        int dx = i % DW - 1, dy = i / DW - 1;
-        long c = (long)wrapi(y0 + dy, a.Ny) * a.P + wrapi(x0 + dx, a.Nx);
-        float r = 1.f / colval<C>(a, 0, z + 1, c);
-        vel[vb2][0][dy + 1][dx + 1] = colval<C>(a, 1, z + 1, c) * r;
-        vel[vb2][1][dy + 1][dx + 1] = colval<C>(a, 2, z + 1, c) * r;
-        vel[vb2][2][dy + 1][dx + 1] = colval<C>(a, 3, z + 1, c) * r;
-        vel[vb2][3][dy + 1][dx + 1] = colval<C>(a, 4, z + 1, c) * r;
+      if (hasR1) {
+        float r = frcp(hv[1][0]);
+        (&vb(0)[0][0])[hd] = hv[1][1] * r;
+        (&vb(1)[0][0])[hd] = hv[1][2] * r;
+        (&vb(2)[0][0])[hd] = hv[1][3] * r;
+        (&vb(3)[0][0])[hd] = hv[1][4] * r;
 #pragma unroll
-        for (int m = 0; m < C::NM; ++m) vel[vb2][4 + m][dy + 1][dx + 1] = colval<C>(a, C::IQV + m, z + 1, c) * r;
+        for (int m = 0; m < C::NM; ++m) (&vb(4 + m)[0][0])[hd] = hv[1][C::IQV + m] * r;
+      }
+      const float idl = 2.f / prof(a, PR_DELTA, z);
+      {
+        float A = win[0][3] * kmo;
+        (&kmp[0][0][0])[od] = A;
+        (&kmp[1][0][0])[od] = A * (1.f + lso * idl);
+      }
+      if (hasR1) {
+        float A = hv[0][0] * kh;
+        (&kmp[0][0][0])[hd] = A;
+        (&kmp[1][0][0])[hd] = A * (1.f + lh * idl);
       }
     }
     __syncthreads();
 
-    /* (2) rho^-1 and p' on [-1,TX]x[-1,TY]; rho*Km, rho*Km*invPr, (2/3)rho*e at level z */
-    for (int i = tid; i < DW * DH; i += NT) {
-      int dx = i % DW, dy = i / DW;
-      float rho = raw[0][dy + HH - 1][dx + HH - 1];
-      float qv = C::NM > 0 ? raw[C::IQV][dy + HH - 1][dx + HH - 1] : 0.f;
-      dvr[dy][dx] = 1.f / rho;
-      dvp[dy][dx] = pprime(a, rthm_of(raw[4][dy + HH - 1][dx + HH - 1], rho, qv), z);
-      if constexpr (C::PHYS) {
-        long c = (long)wrapi(y0 + dy - 1, a.Ny) * a.P + wrapi(x0 + dx - 1, a.Nx);
-        float km = a.km[(long)(z + 1) * a.ps + c], l = a.ls[(long)(z + 1) * a.ps + c];
-        float A = rho * km;
-        kmp[0][dy][dx] = A;
-        kmp[1][dy][dx] = A * (1.f + 2.f * l / prof(a, PR_DELTA, z));
-        kmp[2][dy][dx] = C::TKE ? (2.f / 3.f) * raw[C::IE][dy + HH - 1][dx + HH - 1] : 0.f;
-      }
-    }
-    __syncthreads();
-
-    /* (3) x and y face fluxes of every field; SGS stresses on x and y faces */
+    /* ---- P2: faces ---- */
+    /* horizontal p' differences of the own cell, read here so that the p' plane
+     * is free to be rewritten in the next level's first phase */
+    const float pgx = 0.5f * a.dXi * (dvp[ty + 1][tx + 2] - dvp[ty + 1][tx]);
+    const float pgy = 0.5f * a.dYi * (dvp[ty + 2][tx + 1] - dvp[ty][tx + 1]);
     {
       const float Dz = prof(a, PR_D, z);
-      const float hx = 0.5f * a.dXi * Dz, hy = 0.5f * a.dYi * Dz;
-      for (int i = tid; i < NGX + NGY; i += NT) {
-        if (i < NGX) {
-          int fy = i / (TX + 1), fx = i % (TX + 1);
-          float U = hx * (raw[1][fy + HH][fx + HH - 1] * dvr[fy + 1][fx] + raw[1][fy + HH][fx + HH] * dvr[fy + 1][fx + 1]);
-          FaceLoop<C, 0>::xf(raw, gx, fy, fx, U, omb);
-        } else {
-          int j = i - NGX;
-          int fy = j / TX, fx = j % TX;
-          float V = hy * (raw[2][fy + HH - 1][fx + HH] * dvr[fy][fx + 1] + raw[2][fy + HH][fx + HH] * dvr[fy + 1][fx + 1]);
-          FaceLoop<C, 0>::yf(raw, gy, fy, fx, V, omb);
-        }
+      const float hxD = 0.5f * a.dXi * Dz, hyD = 0.5f * a.dYi * Dz;
+      const float (*dr)[DW] = dvr;
+      /* own x face (low face of the own cell) and own y face */
+      {
+        float U = hxD * (raw[1][ty + HH][tx + HH - 1] * dr[ty + 1][tx] + raw[1][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
+        FaceLoop<C, 0>::xf(raw, gx, ty, tx, U, omb);
+        float V = hyD * (raw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + raw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
+        FaceLoop<C, 0>::yf(raw, gy, ty, tx, V, omb);
       }
-      if constexpr (C::PHYS) {
+      /* ring faces: wave 0 the y face row y=TY, wave 1 (lanes < TY) the x face column x=TX */
+      if (wave == 0) {
+        const int fx = tx, fy = TY;
+        float V = hyD * (raw[2][fy + HH - 1][fx + HH] * dr[fy][fx + 1] + raw[2][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
+        FaceLoop<C, 0>::yf(raw, gy, fy, fx, V, omb);
+      } else if (wave == 1 && tx < TY) {
+        const int fx = TX, fy = tx;
+        float U = hxD * (raw[1][fy + HH][fx + HH - 1] * dr[fy + 1][fx] + raw[1][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
+        FaceLoop<C, 0>::xf(raw, gx, fy, fx, U, omb);
+      }
+      if constexpr (PH) {
         const float J33z = prof(a, PR_J33, z);
-        const int vz = vb1, vm = vb0;
-        for (int i = tid; i < NGX + NGY; i += NT) {
-          bool xface = i < NGX;
-          int fy, fx;
-          if (xface) { fy = i / (TX + 1); fx = i % (TX + 1); }
-          else { int j = i - NGX; fy = j / TX; fx = j % TX; }
-          /* region coordinates of the cell (fx, fy) and its -x / -y neighbours */
+        auto vz = [&](int q) -> const Plane & { return VL(z, q); };
+        auto vm = [&](int q) -> const Plane & { return VL(z - 1, q); };
+        /* face-located SGS stresses (cuda_sgsTurbDevice.cu:60-367) at the cell (fx,fy) */
+        auto xface = [&](int fx, int fy, int c2) {
           const int X = fx + 1, Y = fy + 1;
-#define VL(b, q, yy, xx) vel[b][q][yy][xx]
-          float A = kmp[0][Y][X], Bv = kmp[1][Y][X], E = kmp[2][Y][X];
-          float s11 = a.dXi * (VL(vz, 0, Y, X) - VL(vz, 0, Y, X - 1));
-          float s22 = a.dYi * (VL(vz, 1, Y, X) - VL(vz, 1, Y - 1, X));
-          float s33 = J33z * a.dZi * (VL(vz, 2, Y, X) - VL(vm, 2, Y, X));
-          float div = s11 + s22 + s33;
-          float s21 = 0.25f * (a.dXi * ((VL(vz, 1, Y, X) - VL(vz, 1, Y, X - 1)) + (VL(vz, 1, Y - 1, X) - VL(vz, 1, Y - 1, X - 1)))
-                             + a.dYi * ((VL(vz, 0, Y, X) - VL(vz, 0, Y - 1, X)) + (VL(vz, 0, Y, X - 1) - VL(vz, 0, Y - 1, X - 1))));
-          const long c2 = (long)wrapi(y0 + fy, a.Ny) * a.P + wrapi(x0 + fx, a.Nx);
-          if (xface) {
-            float s31 = 0.25f * (a.dXi * ((VL(vz, 2, Y, X) - VL(vz, 2, Y, X - 1)) + (VL(vm, 2, Y, X) - VL(vm, 2, Y, X - 1)))
-                               + J33z * a.dZi * ((VL(vz, 0, Y, X) - VL(vm, 0, Y, X)) + (VL(vz, 0, Y, X - 1) - VL(vm, 0, Y, X - 1))));
-            tb1[0][fy][fx] = -2.f * A * s11 + (2.f / 3.f) * A * div + E;
-            tb1[1][fy][fx] = -2.f * A * s21;
-            tb1[2][fy][fx] = (z == 0) ? a.s.t13[c2] : -2.f * A * s31;
-            tb1[3][fy][fx] = -Bv * a.dXi * (VL(vz, 3, Y, X) - VL(vz, 3, Y, X - 1));
+          float A = kmp[0][Y][X], Bv = kmp[1][Y][X], E = C::TKE ? (2.f / 3.f) * raw[C::IE][fy + HH][fx + HH] : 0.f;
+          float s11 = a.dXi * (vz(0)[Y][X] - vz(0)[Y][X - 1]);
+          float s22 = a.dYi * (vz(1)[Y][X] - vz(1)[Y - 1][X]);
+          float s33 = J33z * a.dZi * (vz(2)[Y][X] - vm(2)[Y][X]);
+          float s21 = 0.25f * (a.dXi * ((vz(1)[Y][X] - vz(1)[Y][X - 1]) + (vz(1)[Y - 1][X] - vz(1)[Y - 1][X - 1]))
+                             + a.dYi * ((vz(0)[Y][X] - vz(0)[Y - 1][X]) + (vz(0)[Y][X - 1] - vz(0)[Y - 1][X - 1])));
+          float s31 = 0.25f * (a.dXi * ((vz(2)[Y][X] - vz(2)[Y][X - 1]) + (vm(2)[Y][X] - vm(2)[Y][X - 1]))
+                             + J33z * a.dZi * ((vz(0)[Y][X] - vm(0)[Y][X]) + (vz(0)[Y][X - 1] - vm(0)[Y][X - 1])));
+          tb1[0][fy][fx] = -2.f * A * s11 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+          tb1[1][fy][fx] = -2.f * A * s21;
+          tb1[2][fy][fx] = (z == 0) ? a.s.t13[c2] : -2.f * A * s31;
+          tb1[3][fy][fx] = -Bv * a.dXi * (vz(3)[Y][X] - vz(3)[Y][X - 1]);
 #pragma unroll
-            for (int m = 0; m < C::NM; ++m) tb1[4 + m][fy][fx] = -Bv * a.dXi * (VL(vz, 4 + m, Y, X) - VL(vz, 4 + m, Y, X - 1));
-          } else {
-            float s32 = 0.25f * (a.dYi * ((VL(vz, 2, Y, X) - VL(vz, 2, Y - 1, X)) + (VL(vm, 2, Y, X) - VL(vm, 2, Y - 1, X)))
-                               + J33z * a.dZi * ((VL(vz, 1, Y, X) - VL(vm, 1, Y, X)) + (VL(vz, 1, Y - 1, X) - VL(vm, 1, Y - 1, X))));
-            tb2[0][fy][fx] = -2.f * A * s21;
-            tb2[1][fy][fx] = -2.f * A * s22 + (2.f / 3.f) * A * div + E;
-            tb2[2][fy][fx] = (z == 0) ? a.s.t23[c2] : -2.f * A * s32;
-            tb2[3][fy][fx] = -Bv * a.dYi * (VL(vz, 3, Y, X) - VL(vz, 3, Y - 1, X));
+          for (int m = 0; m < C::NM; ++m) tb1[4 + m][fy][fx] = -Bv * a.dXi * (vz(4 + m)[Y][X] - vz(4 + m)[Y][X - 1]);
+        };
+        auto yface = [&](int fx, int fy, int c2) {
+          const int X = fx + 1, Y = fy + 1;
+          float A = kmp[0][Y][X], Bv = kmp[1][Y][X], E = C::TKE ? (2.f / 3.f) * raw[C::IE][fy + HH][fx + HH] : 0.f;
+          float s11 = a.dXi * (vz(0)[Y][X] - vz(0)[Y][X - 1]);
+          float s22 = a.dYi * (vz(1)[Y][X] - vz(1)[Y - 1][X]);
+          float s33 = J33z * a.dZi * (vz(2)[Y][X] - vm(2)[Y][X]);
+          float s21 = 0.25f * (a.dXi * ((vz(1)[Y][X] - vz(1)[Y][X - 1]) + (vz(1)[Y - 1][X] - vz(1)[Y - 1][X - 1]))
+                             + a.dYi * ((vz(0)[Y][X] - vz(0)[Y - 1][X]) + (vz(0)[Y][X - 1] - vz(0)[Y - 1][X - 1])));
+          float s32 = 0.25f * (a.dYi * ((vz(2)[Y][X] - vz(2)[Y - 1][X]) + (vm(2)[Y][X] - vm(2)[Y - 1][X]))
+                             + J33z * a.dZi * ((vz(1)[Y][X] - vm(1)[Y][X]) + (vz(1)[Y - 1][X] - vm(1)[Y - 1][X])));
+          tb2[0][fy][fx] = -2.f * A * s21;
+          tb2[1][fy][fx] = -2.f * A * s22 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+          tb2[2][fy][fx] = (z == 0) ? a.s.t23[c2] : -2.f * A * s32;
+          tb2[3][fy][fx] = -Bv * a.dYi * (vz(3)[Y][X] - vz(3)[Y - 1][X]);
 #pragma unroll
-            for (int m = 0; m < C::NM; ++m) tb2[4 + m][fy][fx] = -Bv * a.dYi * (VL(vz, 4 + m, Y, X) - VL(vz, 4 + m, Y - 1, X));
+          for (int m = 0; m < C::NM; ++m) tb2[4 + m][fy][fx] = -Bv * a.dYi * (vz(4 + m)[Y][X] - vz(4 + m)[Y - 1][X]);
+        };
+        xface(tx, ty, col);
+        yface(tx, ty, col);
+        if (wave == 2) {
+          yface(tx, TY, z == 0 ? wrapi(y0 + TY, a.Ny) * a.P + cx : 0);
+        } else if (wave == 3 && tx < TY) {
+          xface(TX, tx, z == 0 ? wrapi(y0 + tx, a.Ny) * a.P + wrapi(x0 + TX, a.Nx) : 0);
+        }
+        if constexpr (C::TKE) {
+          /* cell-centred TKE transport fluxes rho*Km*de/dx, rho*Km*de/dy (cuda_sgstkeDevice.cu:309-397) */
+          auto ecell = [&](int lx, int ly) { return raw[C::IE][ly + HH][lx + HH] * frcp(raw[0][ly + HH][lx + HH]); };
+          auto qxf = [&](int lx, int ly) { return 0.5f * a.dXi * (ecell(lx + 1, ly) - ecell(lx - 1, ly)) * kmp[0][ly + 1][lx + 1]; };
+          auto qyf = [&](int lx, int ly) { return 0.5f * a.dYi * (ecell(lx, ly + 1) - ecell(lx, ly - 1)) * kmp[0][ly + 1][lx + 1]; };
+          qx[ty][tx + 1] = qxf(tx, ty);
+          qy[ty + 1][tx] = qyf(tx, ty);
+          if (wave == 4) qy[0][tx] = qyf(tx, -1);
+          else if (wave == 5) qy[TY + 1][tx] = qyf(tx, TY);
+          else if (wave == 6 && tx < 2 * TY) {
+            int ly = tx % TY, side = tx / TY;
+            if (side == 0) qx[ly][0] = qxf(-1, ly);
+            else qx[ly][TX + 1] = qxf(TX, ly);
           }
-#undef VL
         }
       }
     }
     __syncthreads();
 
-    /* (4) own cell: tendencies and the stage update */
+    /* ---- P3: own cell ---- */
     {
       const float Dz = prof(a, PR_D, z), Dp = prof(a, PR_D, z + 1);
       const float J33z = prof(a, PR_J33, z), J33p = prof(a, PR_J33, z + 1);
       const float invD = prof(a, PR_INVD, z);
-      /* z face z+1/2 (zero at the lid) */
-      float rinp = 1.f / win[0][4];
+      float rinp = frcp(win[0][4]);
       float W = (z + 1 >= Nz) ? 0.f : 0.5f * a.dZi * (Dp * J33p * win[3][4] * rinp + Dz * J33z * win[3][3] * rin);
       float gzhi[NF];
       FaceLoop<C, 0>::zf(win, gzhi, W, omb);
@@ -576,12 +675,9 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         F[f] = -invD * ((gx[f][ty][tx + 1] - gx[f][ty][tx]) + (gy[f][ty + 1][tx] - gy[f][ty][tx]) + (gzhi[f] - gzlo[f]));
-
       const float rho = win[0][3];
       const int X = tx + 1, Y = ty + 1;
-      /* pressure gradient (cuda_pressureDevice.cu:118-194) */
       {
-        float pzp = pz[2];
         float fac = 1.f;
         if constexpr (C::NM > 0) {
           float qs = 0.f;
@@ -589,11 +685,10 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
           for (int m = 0; m < C::NM; ++m) qs += win[C::IQV + m][3];
           fac = 1.f / (1.f + qs * rin * 1e-3f);
         }
-        F[1] -= fac * 0.5f * a.dXi * (dvp[Y][X + 1] - dvp[Y][X - 1]);
-        F[2] -= fac * 0.5f * a.dYi * (dvp[Y + 1][X] - dvp[Y - 1][X]);
-        F[3] -= fac * 0.5f * a.dZi * J33z * (pzp - pz[0]);
+        F[1] -= fac * pgx;
+        F[2] -= fac * pgy;
+        F[3] -= fac * 0.5f * a.dZi * J33z * (pz[2] - pz[0]);
       }
-      /* Rayleigh damping and buoyancy on rho*w (cuda_rayleighDampingDevice.cu:49-79, cuda_buoyancyDevice.cu:48-67) */
       if (a.damp) F[3] -= prof(a, PR_DAMP, z) * win[3][3];
       if constexpr (C::NM == 0) {
         F[3] -= a.g * (rho - prof(a, PR_RHO, z));
@@ -603,14 +698,12 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
         for (int m = 0; m < C::NM; ++m) rt += win[C::IQV + m][3] * 1e-3f;
         F[3] -= (rho / rt) * a.g * (rt - prof(a, PR_RHO, z));
       }
-      /* Coriolis with geostrophic forcing (cuda_coriolisDevice.cu:50-58) */
       {
         float u = win[1][3] * rin, v = win[2][3] * rin, w = win[3][3] * rin;
         F[1] += a.fH * (v - prof(a, PR_VG, z)) - a.fV * w;
         F[2] -= a.fH * (u - prof(a, PR_UG, z));
         F[3] += a.fV * (u - prof(a, PR_UG, z));
       }
-      /* 6th-order vertical filter on rho*w (cuda_filtersDevice.cu:75-122) */
       if (a.filt != 0.f) {
         const float *w = win[3];
         float gm = w[3] - w[2], gp = w[4] - w[3];
@@ -621,32 +714,30 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
         F[3] += a.filt * (fp - fm);
       }
 
-      if constexpr (C::PHYS) {
-        /* z-face stresses at z+1 for the own column */
+      if constexpr (PH) {
+        auto vz = [&](int q) -> const Plane & { return VL(z, q); };
+        auto vp = [&](int q) -> const Plane & { return VL(z + 1, q); };
+        auto vm = [&](int q) -> const Plane & { return VL(z - 1, q); };
         float zn[4 + C::NM];
         {
-          const int vz = vb1, vp = vb2;
-          float A = kmn * win[0][4];
-          float lsd = prof(a, PR_DELTA, z + 1);
-          float Bv = A * (1.f + 2.f * lsn / lsd);
+          float A = kmo1 * win[0][4];
+          float Bv = A * (1.f + 2.f * lso1 / prof(a, PR_DELTA, z + 1));
           float E = C::TKE ? (2.f / 3.f) * win[C::IE][4] : 0.f;
-          float wpY = vel[vp][2][Y][X], wzY = vel[vz][2][Y][X];
-          float s33 = J33p * a.dZi * (wpY - wzY);
-          float s11 = a.dXi * (vel[vp][0][Y][X] - vel[vp][0][Y][X - 1]);
-          float s22 = a.dYi * (vel[vp][1][Y][X] - vel[vp][1][Y - 1][X]);
-          float s31 = 0.25f * (a.dXi * ((vel[vp][2][Y][X] - vel[vp][2][Y][X - 1]) + (vel[vz][2][Y][X] - vel[vz][2][Y][X - 1]))
-                             + J33p * a.dZi * ((vel[vp][0][Y][X] - vel[vz][0][Y][X]) + (vel[vp][0][Y][X - 1] - vel[vz][0][Y][X - 1])));
-          float s32 = 0.25f * (a.dYi * ((vel[vp][2][Y][X] - vel[vp][2][Y - 1][X]) + (vel[vz][2][Y][X] - vel[vz][2][Y - 1][X]))
-                             + J33p * a.dZi * ((vel[vp][1][Y][X] - vel[vz][1][Y][X]) + (vel[vp][1][Y - 1][X] - vel[vz][1][Y - 1][X])));
+          float s33 = J33p * a.dZi * (vp(2)[Y][X] - vz(2)[Y][X]);
+          float s11 = a.dXi * (vp(0)[Y][X] - vp(0)[Y][X - 1]);
+          float s22 = a.dYi * (vp(1)[Y][X] - vp(1)[Y - 1][X]);
+          float s31 = 0.25f * (a.dXi * ((vp(2)[Y][X] - vp(2)[Y][X - 1]) + (vz(2)[Y][X] - vz(2)[Y][X - 1]))
+                             + J33p * a.dZi * ((vp(0)[Y][X] - vz(0)[Y][X]) + (vp(0)[Y][X - 1] - vz(0)[Y][X - 1])));
+          float s32 = 0.25f * (a.dYi * ((vp(2)[Y][X] - vp(2)[Y - 1][X]) + (vz(2)[Y][X] - vz(2)[Y - 1][X]))
+                             + J33p * a.dZi * ((vp(1)[Y][X] - vz(1)[Y][X]) + (vp(1)[Y - 1][X] - vz(1)[Y - 1][X])));
           zn[0] = -2.f * A * s31;
           zn[1] = -2.f * A * s32;
           zn[2] = -2.f * A * s33 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
-          zn[3] = -Bv * J33p * a.dZi * (vel[vp][3][Y][X] - vel[vz][3][Y][X]);
+          zn[3] = -Bv * J33p * a.dZi * (vp(3)[Y][X] - vz(3)[Y][X]);
 #pragma unroll
-          for (int m = 0; m < C::NM; ++m) zn[4 + m] = -Bv * J33p * a.dZi * (vel[vp][4 + m][Y][X] - vel[vz][4 + m][Y][X]);
+          for (int m = 0; m < C::NM; ++m) zn[4 + m] = -Bv * J33p * a.dZi * (vp(4 + m)[Y][X] - vz(4 + m)[Y][X]);
           Aown[2] = A;
         }
-        /* divergence of the SGS stresses (cuda_sgsTurbDevice.cu:447-547) */
         const float zdz = J33z * a.dZi;
         F[1] -= a.dXi * (tb1[0][ty][tx + 1] - tb1[0][ty][tx]) + a.dYi * (tb2[0][ty + 1][tx] - tb2[0][ty][tx]) + zdz * (zn[0] - zc[0]);
         F[2] -= a.dXi * (tb1[1][ty][tx + 1] - tb1[1][ty][tx]) + a.dYi * (tb2[1][ty + 1][tx] - tb2[1][ty][tx]) + zdz * (zn[1] - zc[1]);
@@ -657,48 +748,38 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
           F[C::IQV + m] -= a.dXi * (tb1[4 + m][ty][tx + 1] - tb1[4 + m][ty][tx]) + a.dYi * (tb2[4 + m][ty + 1][tx] - tb2[4 + m][ty][tx])
                           + zdz * (zn[4 + m] - zc[4 + m]);
         if constexpr (C::TKE) {
-          /* prognostic SGS TKE sources (cuda_sgstkeDevice.cu:155-397) */
           const float rhoe = win[C::IE][3];
           const float e = rhoe * rin;
           const float th = win[4][3] * rin;
           float src = a.g / th * zc[3];
           {
-            float ce = 1.9f * a.c_k + (0.93f - 1.9f * a.c_k) * lsz / prof(a, PR_DELTA, z);
-            src += -ce * rhoe * sqrtf(e) / lsz;
+            float ce = 1.9f * a.c_k + (0.93f - 1.9f * a.c_k) * lso / prof(a, PR_DELTA, z);
+            src += -ce * rhoe * sqrtf(e) / lso;
           }
           {
-            const int vz = vb1;
-            float gxu = 0.5f * a.dXi * (vel[vz][0][Y][X + 1] - vel[vz][0][Y][X - 1]);
-            float gyu = 0.5f * a.dYi * (vel[vz][0][Y + 1][X] - vel[vz][0][Y - 1][X]);
-            float gzu = 0.5f * zdz * (vel[vb2][0][Y][X] - vel[vb0][0][Y][X]);
-            float gxv = 0.5f * a.dXi * (vel[vz][1][Y][X + 1] - vel[vz][1][Y][X - 1]);
-            float gyv = 0.5f * a.dYi * (vel[vz][1][Y + 1][X] - vel[vz][1][Y - 1][X]);
-            float gzv = 0.5f * zdz * (vel[vb2][1][Y][X] - vel[vb0][1][Y][X]);
-            float gxw = 0.5f * a.dXi * (vel[vz][2][Y][X + 1] - vel[vz][2][Y][X - 1]);
-            float gyw = 0.5f * a.dYi * (vel[vz][2][Y + 1][X] - vel[vz][2][Y - 1][X]);
-            float gzw = 0.5f * zdz * (vel[vb2][2][Y][X] - vel[vb0][2][Y][X]);
+            float gxu = 0.5f * a.dXi * (vz(0)[Y][X + 1] - vz(0)[Y][X - 1]);
+            float gyu = 0.5f * a.dYi * (vz(0)[Y + 1][X] - vz(0)[Y - 1][X]);
+            float gzu = 0.5f * zdz * (vp(0)[Y][X] - vm(0)[Y][X]);
+            float gxv = 0.5f * a.dXi * (vz(1)[Y][X + 1] - vz(1)[Y][X - 1]);
+            float gyv = 0.5f * a.dYi * (vz(1)[Y + 1][X] - vz(1)[Y - 1][X]);
+            float gzv = 0.5f * zdz * (vp(1)[Y][X] - vm(1)[Y][X]);
+            float gxw = 0.5f * a.dXi * (vz(2)[Y][X + 1] - vz(2)[Y][X - 1]);
+            float gyw = 0.5f * a.dYi * (vz(2)[Y + 1][X] - vz(2)[Y - 1][X]);
+            float gzw = 0.5f * zdz * (vp(2)[Y][X] - vm(2)[Y][X]);
             float t11 = tb1[0][ty][tx], t21 = tb1[1][ty][tx], t31 = tb1[2][ty][tx];
             float t22 = tb2[1][ty][tx], t32 = tb2[2][ty][tx], t33 = zc[2];
             src -= t11 * gxu + t21 * gyu + t31 * gzu + t21 * gxv + t22 * gyv + t32 * gzv + t31 * gxw + t32 * gyw + t33 * gzw;
           }
           {
-            /* turbulent transport: d/dx_j (rho Km de/dx_j), centred gradients */
-            const int R = ty + HH, Cc = tx + HH;
-            auto ecell = [&](int yy, int xx) { return raw[C::IE][yy][xx] / raw[0][yy][xx]; };
-            float ex2p = ecell(R, Cc + 2), ex2m = ecell(R, Cc - 2), ey2p = ecell(R + 2, Cc), ey2m = ecell(R - 2, Cc);
-            float dexp = 0.5f * a.dXi * (ex2p - e), dexm = 0.5f * a.dXi * (e - ex2m);
-            float deyp = 0.5f * a.dYi * (ey2p - e), deym = 0.5f * a.dYi * (e - ey2m);
             float ezp2 = win[C::IE][5] * riz[4], ezm2 = win[C::IE][1] * riz[0];
             float dezp = 0.5f * J33p * a.dZi * (ezp2 - e);
             float dezm = 0.5f * prof(a, PR_J33, z - 1) * a.dZi * (e - ezm2);
-            src += a.dXi * (dexp * kmp[0][Y][X + 1] - dexm * kmp[0][Y][X - 1])
-                 + a.dYi * (deyp * kmp[0][Y + 1][X] - deym * kmp[0][Y - 1][X])
+            src += a.dXi * (qx[ty][tx + 2] - qx[ty][tx]) + a.dYi * (qy[ty + 2][tx] - qy[ty][tx])
                  + zdz * (dezp * Aown[2] - dezm * Aown[0]);
           }
           F[C::IE] += src;
         }
         if constexpr (C::NM > 1) {
-          /* saturation adjustment and latent heating (cuda_moistureDevice.cu:98-245) */
           if (a.mpActive) {
             const float rth = win[4][3];
             const float thd = rth * rin;
@@ -730,7 +811,6 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
             F[4] += ((thd * a.L_v) / (Tf * a.cp)) * fc;
           }
         }
-        /* carry z-face stresses and rho*Km up one level */
 #pragma unroll
         for (int q = 0; q < 4 + C::NM; ++q) zc[q] = zn[q];
         Aown[0] = Aown[1];
@@ -738,17 +818,17 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
       }
 
       if (active) {
-        const long o = (long)z * a.ps + col;
+        const int o = z * ps + col;
         if (a.emitF) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) a.out[f * a.fs + o] = F[f];
+          for (int f = 0; f < NF; ++f) a.out[(long)f * a.fs + o] = F[f];
         } else {
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
             float v = fmaf(a.coefdt, F[f], bse[f]);
             if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
             if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
-            a.out[f * a.fs + o] = v;
+            a.out[(long)f * a.fs + o] = v;
           }
         }
         if (a.pout) a.pout[o] = pz[1];
@@ -758,26 +838,30 @@ __global__ void __launch_bounds__(NT) stage_kernel(StepArgs a) {
       rin = rinp;
     }
 
-    /* (5) advance the column window and the own-column pressure */
+    /* ---- advance the pipelines ---- */
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
 #pragma unroll
       for (int j = 0; j < 6; ++j) win[f][j] = win[f][j + 1];
       win[f][6] = nxt[f];
     }
+#pragma unroll
+    for (int l = 0; l < PD; ++l)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) hv[l][f] = hv[l + 1][f];
     pz[0] = pz[1];
     pz[1] = pz[2];
     if (z + 2 < Nz) {
-      float qv = C::NM > 0 ? win[C::IQV][4] : 0.f;
-      pz[2] = pprime(a, rthm_of(win[4][4], win[0][4], qv), z + 2);
+      float q = C::NM > 0 ? win[C::IQV][4] : 0.f;
+      pz[2] = pprime(a, rthm_of(win[4][4], win[0][4], q), z + 2);
     }
-    if constexpr (C::PHYS) {
+    if constexpr (PH) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) riz[j] = riz[j + 1];
-      riz[4] = 1.f / win[0][5];
-      int t = vb0; vb0 = vb1; vb1 = vb2; vb2 = t;
+      riz[4] = frcp(win[0][5]);
+      kmo = kmo1; lso = lso1; kmo1 = kmo2; lso1 = lso2;
+      kh = khn; lh = lhn;
     }
-    __syncthreads();
   }
 }
 

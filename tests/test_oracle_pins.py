@@ -4,11 +4,14 @@ The reference ships no tests or golden vectors for its time step, and its
 executable cannot be built here (needs nvcc, MPI and netCDF), so every oracle
 function is pinned by mathematics:
   RK3-WS2002 amplification factor; advection exactness on polynomials and
-  convergence order; discrete mass conservation; geostrophic/hydrostatic rest
-  gives F == 0; Rayleigh sin^2 profile; neutral surface-layer drag
-  coefficient and Businger-Dyer unstable functions; SGS length-scale branches,
-  pure-shear stress and N^2 of a linear theta; Flatau saturation vapour
-  pressure values; FP32 vs FP64 agreement.
+  the convergence order of every scheme for both wind signs; discrete mass
+  conservation; geostrophic/hydrostatic rest gives F == 0; f-plane Coriolis,
+  Archimedes buoyancy and the ideal-gas pressure-gradient force; Rayleigh
+  sin^2 profile; neutral surface-layer drag coefficient and Businger-Dyer
+  unstable functions; SGS length-scale branches, pure-shear stress, N^2 of a
+  linear theta and Moeng's TKE dissipation; Flatau saturation vapour pressure
+  values, the saturation threshold and the water/heat budgets of saturation
+  adjustment; FP32 vs FP64 agreement.
 """
 import math
 
@@ -114,6 +117,46 @@ def test_advection_convergence_order(scheme, bhyb, order):
     e1, e2 = _sine_error(32, scheme, bhyb), _sine_error(64, scheme, bhyb)
     rate = math.log2(e1 / e2)
     assert rate > order - 0.35, (rate, e1, e2)
+
+
+def _monotone_error(n, scheme, U):
+    """Max error of -d(U phi)/dx for phi = 300 + exp(2x/L), over the middle half of the domain
+    (far from the periodic seam, where the data is smooth and monotone)."""
+    L = 1000.0
+    dx = L / n
+    p = small(Nx=n, Ny=4, Nz=4, d_xi=dx, advectionSelector=scheme, b_hyb=0.0, pgfSelector=0, buoyancySelector=0,
+              coriolisSelector=0, turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
+    x = (np.arange(n + 2 * NH) - NH + 0.5) * dx
+    o = fe_py.Oracle(p, 64)
+    h = o.arr("HYDRO")
+    h[0] = 1.0
+    h[1] = U
+    h[2] = 0.0
+    h[3] = 0.0
+    h[4] = (300.0 + np.exp(2 * x / L))[:, None, None]
+    o.arr("BS")[0] = 1.0
+    o.arr("BS")[1] = 300.0
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    o.build_frhs(0)
+    F = o.arr("HYDRO_F")[4][:, NH, NH].copy()
+    o.close()
+    exact = -U * (2 / L) * np.exp(2 * x / L)
+    sel = slice(NH + n // 4, NH + 3 * n // 4)
+    return np.abs(F[sel] - exact[sel]).max()
+
+
+@pytest.mark.parametrize("U", [2.0, -2.0])
+@pytest.mark.parametrize("scheme,order", [(0, 1), (1, 2), (2, 3), (3, 5), (4, 3), (5, 5), (6, 2)])
+def test_advection_order_all_schemes_both_winds(scheme, order, U):
+    """Finite-difference order of every advection scheme on smooth monotone data, for both wind
+    signs: 1st-order upstream 1; QUICK face values are 3rd order but their flux difference is
+    2nd order for point values; hybrid 3rd/4th (b=0) 3; hybrid 5th/6th (b=0) 5; WENO3 3 and
+    WENO5 5 away from critical points (Jiang & Shu 1996); centred 2.  A wrong upwind branch or
+    stencil coefficient drops the order."""
+    e1, e2 = _monotone_error(32, scheme, U), _monotone_error(64, scheme, U)
+    rate = math.log2(e1 / e2)
+    assert abs(rate - order) < 0.3, (rate, e1, e2)
 
 
 def test_constant_fields_have_no_advective_tendency():
@@ -350,3 +393,146 @@ def test_fp32_oracle_tracks_fp64(tmp_path):
         assert np.all(rel < 2e-4), rel
     o32.close()
     o64.close()
+
+
+def _uniform_oracle(p, rho=1.0, rth=300.0):
+    """FP64 oracle on a flat unit-metric grid with a uniform base state."""
+    o = fe_py.Oracle(p, 64)
+    for a in ("J33", "DJAC", "INVDJAC"):
+        o.arr(a)[...] = 1.0
+    nz = p["Nz"] + 2 * NH
+    o.arr("ZPOS")[0] = ((np.arange(nz) - NH + 0.5) * p["d_zeta"])[None, None, :]
+    h = o.arr("HYDRO")
+    h[0] = rho
+    h[1:4] = 0.0
+    h[4] = rth
+    o.arr("BS")[0] = rho
+    o.arr("BS")[1] = rth
+    return o
+
+
+def test_coriolis_inertial_tendency():
+    """f-plane: a uniform ageostrophic wind (du, dv) at unit density feels
+    F_u = f dv, F_v = -f du with f = 2 Omega sin(latitude), 2 Omega = 1.45842e-4 1/s;
+    the geostrophic wind (U_g, V_g) itself is unforced."""
+    lat = 43.2885
+    p = small(surflayerSelector=0, dampingLayerSelector=0, coriolisSelector=1, coriolisLatitude=lat)
+    o = _uniform_oracle(p)
+    du, dv = 1.5, -2.0
+    h = o.arr("HYDRO")
+    h[1] = p["U_g"] + du
+    h[2] = p["V_g"] + dv
+    o.arr("TKE")[0] = 0.0
+    o.build_frhs(0)
+    f = 1.45842e-4 * math.sin(math.radians(lat))
+    F = I(o.arr("HYDRO_F"))
+    np.testing.assert_allclose(F[1], f * dv, rtol=2e-7)
+    np.testing.assert_allclose(F[2], -f * du, rtol=2e-7)
+    assert np.all(F[3] == 0.0) and np.all(F[0] == 0.0)
+    o.close()
+
+
+def test_buoyancy_archimedes():
+    """A density excess d over the hydrostatic base state at unchanged rho*theta (so p' = 0)
+    gives F_rhow = -g d and nothing else."""
+    p = small(surflayerSelector=0, dampingLayerSelector=0, coriolisSelector=0)
+    o = _uniform_oracle(p)
+    d = 0.012
+    o.arr("HYDRO")[0] = 1.0 + d
+    o.arr("TKE")[0] = 0.0
+    o.build_frhs(0)
+    F = I(o.arr("HYDRO_F"))
+    np.testing.assert_allclose(F[3], -float(np.float32(9.81)) * d, rtol=1e-6)
+    assert np.all(F[[0, 1, 2, 4]] == 0.0)
+    o.close()
+
+
+def _pgf_error(n):
+    L = 1000.0
+    dx = L / n
+    p = small(Nx=n, Ny=4, Nz=4, d_xi=dx, surflayerSelector=0, dampingLayerSelector=0, coriolisSelector=0,
+              buoyancySelector=0, turbulenceSelector=0, TKESelector=0)
+    o = _uniform_oracle(p)
+    x = (np.arange(n + 2 * NH) - NH + 0.5) * dx
+    rth = 300.0 * (1.0 + 1e-3 * np.sin(2 * np.pi * x / L))
+    o.arr("HYDRO")[4] = rth[:, None, None]
+    o.build_frhs(0)
+    Fu = o.arr("HYDRO_F")[1][NH:-NH, NH, NH].copy()
+    o.close()
+    f32 = lambda v: float(np.float32(v))
+    R, cv, p0 = f32(287.04), f32(718.0), 1.0e5
+    cp = R + cv
+    gam = cp / cv
+    C = R / p0 ** (R / cp)
+    xe = x[NH:-NH]
+    rte = rth[NH:-NH]
+    # ideal gas: p = (C rho theta)^gamma, so dp'/dx = gamma C^gamma (rho theta)^(gamma-1) d(rho theta)/dx
+    dpdx = gam * C ** gam * rte ** (gam - 1) * 300.0 * 1e-3 * (2 * np.pi / L) * np.cos(2 * np.pi * xe / L)
+    return np.abs(Fu + dpdx).max(), np.abs(dpdx).max()
+
+
+def test_pressure_gradient_force_ideal_gas():
+    """F_rhou = -dp'/dx for p' from the ideal-gas law p = p0 (R rho theta / p0)^(cp/cv):
+    2nd-order convergence, and within 1% at 32 cells per wavelength."""
+    e1, s = _pgf_error(32)
+    e2, _ = _pgf_error(64)
+    assert abs(math.log2(e1 / e2) - 2.0) < 0.1, (e1, e2)
+    assert e1 < 1e-2 * s
+
+
+def test_tke_dissipation_moeng():
+    """Neutral, quiescent column with uniform e: l = Delta and the only TKE tendency is
+    dissipation -C_eps rho e^1.5 / l with C_eps = 0.19 + 0.74 l/Delta = 0.93 (Moeng 1984)."""
+    p = small(Nx=8, Ny=8, Nz=12, surflayerSelector=0, coriolisSelector=0, dampingLayerSelector=0,
+              d_xi=10.0, d_eta=10.0, d_zeta=10.0, c_k=0.10)
+    o = _uniform_oracle(p, rho=1.1, rth=1.1 * 300.0)
+    e0 = 0.4
+    o.arr("TKE")[0] = 1.1 * e0
+    o.build_frhs(2)
+    Fe = o.arr("TKE_F")[0][NH:-NH, NH:-NH, NH:NH + p["Nz"] - 2]
+    np.testing.assert_allclose(Fe, -0.93 * 1.1 * e0 ** 1.5 / 10.0, rtol=1e-6)
+    o.close()
+
+
+def test_saturation_adjustment_threshold_and_budgets(tmp_path):
+    """Saturation adjustment (moistureCond = 3, base-state pressure): condensation starts where
+    rho q_v crosses rho r_s with r_s = eps e_s / (p - e_s) from the tabulated e_s
+    (Flatau 1992 polynomial, itself pinned above); condensate is conserved
+    (F_qv + F_ql = 0) and heats at F_rhotheta = (theta / T) (L_v / c_p) f_cond."""
+    p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=10, thetaPerturbationSwitch=0, surflayerSelector=0, coriolisSelector=0,
+             dampingLayerSelector=0)
+    m = host_model(p, tmp_path)
+    o = fe_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    f32 = lambda v: float(np.float32(v))
+    R, Rv, cv, Lv, p0 = f32(287.04), f32(461.6), f32(718.0), f32(2.5e6), 1.0e5
+    cp = R + cv
+    h = o.arr("HYDRO")
+    h[1:4] = 0.0
+    h[0] = o.arr("BS")[0]
+    h[4] = o.arr("BS")[1]
+    o.arr("TKE")[0] = 0.0
+    rho, rth = h[0], h[4]
+    pres = (R * rth / p0 ** (R / cp)) ** (cp / cv)
+    T = (rth / rho) * (pres / p0) ** (R / cp)
+    a = [6.11239921, 0.443987641, 0.142986287e-1, 0.264847430e-3, 0.302950461e-5, 0.206739458e-7,
+         0.640689451e-10, -0.952447341e-13, -0.976195544e-15]
+    tc = T - 273.15
+    es = sum(c * tc ** n for n, c in enumerate(a)) * 100.0
+    rs = (R / Rv) * es / (pres - es)
+    fac = np.where(np.arange(h.shape[1]) % 2 == 0, 0.99, 1.01)[:, None, None]
+    mq = o.arr("MOIST")
+    mq[0] = rho * rs * 1e3 * fac
+    mq[1] = 0.0
+    o.build_frhs(2)
+    fc = I(o.arr("FCOND")).reshape(I(h[4]).shape)
+    below = I(np.broadcast_to(fac, h[4].shape)) < 1.0
+    assert np.all(fc[below] == 0.0)
+    assert np.all(fc[~below] > 0.0)
+    Fm = I(o.arr("MOIST_F"))
+    np.testing.assert_allclose(Fm[0] + Fm[1], 0.0, atol=1e-12)
+    np.testing.assert_allclose(Fm[1], 1e3 * fc, rtol=1e-12)
+    Fth = I(o.arr("HYDRO_F"))[4]
+    np.testing.assert_allclose(Fth, I(rth / rho / T) * Lv / cp * fc, rtol=1e-9, atol=1e-15)
+    o.close()

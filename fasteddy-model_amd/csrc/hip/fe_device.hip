@@ -27,6 +27,11 @@
 
 using namespace fe;
 
+/* The one host global this layer reads that the reference's host modules do not
+ * define.  This weak default (the well-conditioned p' form) applies when the
+ * layer is linked under those modules; the host module here overrides it. */
+extern "C" __attribute__((weak)) int amd_pressureForm = 1;
+
 #define HIPCHK(x)                                                                               \
   do {                                                                                          \
     hipError_t e_ = (x);                                                                        \

@@ -3,7 +3,7 @@
  *
  * One launch evaluates the complete tendency of every prognostic field at
  * every cell and applies the RK3-WS2002 stage update (or, in emit mode, stores
- * the tendency).  A 64x4 work-group owns a tile of columns and marches up the
+ * the tendency).  A 64x8 work-group owns a tile of columns and marches up the
  * column:
  *   - each thread keeps its own column's 7-deep window (z-3..z+3) of every
  *     field in registers (vertical stencils, z face fluxes computed once and
@@ -332,6 +332,10 @@ __device__ __forceinline__ void halo_cell(int h, int &lx, int &ly) {
   }
 }
 
+#ifndef FE_PLAIN_WPE
+#define FE_PLAIN_WPE 4   /* occupancy hint (waves per SIMD) of the stage kernel without SGS physics */
+#endif
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 /* One fused RK stage.  Per level z, with two work-group barriers:
@@ -349,7 +353,7 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
  * the next P1 rotate over four levels (u, v, w: levels z-1..z+1 are live) or
  * three (theta and moisture: z, z+1). */
 template <class C>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS ? 2 : 4, 8))) stage_kernel(StepArgs a) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS ? 2 : FE_PLAIN_WPE, 8))) stage_kernel(StepArgs a) {
   constexpr int NF = C::NF;
   constexpr bool PH = C::PHYS;
   constexpr int PD = PH ? 2 : 1;                 /* halo prefetch distance (levels) */
@@ -385,6 +389,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   const int col = cy * a.P + cx;
   const int Nz = a.Nz;
   const int ps = (int)a.ps;
+  const bool baseIn = (a.base == a.in);             /* first RK stage: phi^n is the stage input */
   const float *fin[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) fin[f] = a.in + (long)f * a.fs;
@@ -512,7 +517,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
       for (int f = 0; f < NF; ++f) hv[PD][f] = cv(f, z + PD, hcol);
     }
-    if (!a.emitF) {
+    if (!a.emitF && !baseIn) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) bse[f] = a.base[(long)f * a.fs + z * ps + col];
     }
@@ -825,7 +830,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         } else {
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
-            float v = fmaf(a.coefdt, F[f], bse[f]);
+            float v = fmaf(a.coefdt, F[f], baseIn ? win[f][3] : bse[f]);
             if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
             if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
             a.out[(long)f * a.fs + o] = v;

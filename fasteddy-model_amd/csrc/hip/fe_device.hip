@@ -380,6 +380,12 @@ int cuda_gridInit(void) {
   D.ps = (long)D.Ny * D.P;
   D.fs = D.ps * D.Nz;
   D.np = Nzp + 10;
+  /* the kernels address within one field (and the Nz+2-level SGS arrays) with
+   * 32-bit byte offsets */
+  if ((D.Nz + 2) * D.ps * (long)sizeof(float) > 0xFFFFFFFFL) {
+    fprintf(stderr, "cuda_gridInit: %d x %d x %d exceeds 4 GiB per field\n", Nxp, Nyp, Nzp);
+    return FE_ERR_UNSUPPORTED;
+  }
   D.n3h = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh);
   D.n2h = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh);
   const int i0 = iMin, j0 = jMin;

@@ -39,7 +39,11 @@ __device__ __forceinline__ int wrapi(int v, int n) {
   return v < 0 ? v + n : v;
 }
 
-__device__ __forceinline__ float prof(const StepArgs &a, int p, int z) { return a.prof[p * a.np + z + 3]; }
+/* Profiles are read-only for the whole step and indexed by wave-uniform (p, z):
+ * reading them through the constant address space makes them scalar loads,
+ * which wait on their own counter instead of behind the prefetched vector loads. */
+typedef const __attribute__((address_space(4))) float cfloat;
+__device__ __forceinline__ float prof(const StepArgs &a, int p, int z) { return ((cfloat *)a.prof)[p * a.np + z + 3]; }
 
 /* ---- advective face fluxes: u * f(face) from phi[i-3..i+2] (cuda_advectionDevice.cu:123-926) ---- */
 __device__ __forceinline__ float weno5_side(float a, float b, float c, float d, float e) {
@@ -333,7 +337,7 @@ __device__ __forceinline__ void halo_cell(int h, int &lx, int &ly) {
 }
 
 #ifndef FE_PLAIN_WPE
-#define FE_PLAIN_WPE 4   /* occupancy hint (waves per SIMD) of the stage kernel without SGS physics */
+#define FE_PLAIN_WPE 2   /* occupancy hint (waves per SIMD) of the stage kernel without SGS physics */
 #endif
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -356,7 +360,14 @@ template <class C>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS ? 2 : FE_PLAIN_WPE, 8))) stage_kernel(StepArgs a) {
   constexpr int NF = C::NF;
   constexpr bool PH = C::PHYS;
-  constexpr int PD = PH ? 2 : 1;                 /* halo prefetch distance (levels) */
+  /* Prefetch distances in levels: the own column's level z+OD and the halo's
+   * level z+PD are loaded while level z is computed.  The stage without SGS
+   * physics loads both at z+3, so that a tile's halo lines are fetched at
+   * about the time its neighbours fetch the same lines as their own columns
+   * and are served from L2; the physics stage (register bound) keeps the halo
+   * two levels ahead. */
+  constexpr int OD = PH ? 4 : 3;
+  constexpr int PD = PH ? 2 : 3;
   constexpr int NV = PH ? C::NV : 1;
   constexpr int NB = PH ? C::NB : 1;
   __shared__ float raw[NF][RH][RW];
@@ -388,11 +399,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   const int cx = wrapi(x0 + tx, a.Nx), cy = wrapi(y0 + ty, a.Ny);
   const int col = cy * a.P + cx;
   const int Nz = a.Nz;
-  const int ps = (int)a.ps;
   const bool baseIn = (a.base == a.in);             /* first RK stage: phi^n is the stage input */
-  const float *fin[NF];
+  /* Global accesses use a wave-uniform per-field base pointer plus one 32-bit
+   * byte offset per thread (a field is < 4 GiB), so one offset register serves
+   * every field and the loads take the scalar-base addressing form. */
+  const char *fin[NF], *fbs[NF];
+  char *fout[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) fin[f] = a.in + (long)f * a.fs;
+  for (int f = 0; f < NF; ++f) {
+    fin[f] = (const char *)(a.in + (long)f * a.fs);
+    fbs[f] = (const char *)(a.base + (long)f * a.fs);
+    fout[f] = (char *)(a.out + (long)f * a.fs);
+  }
+  const unsigned pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
+  auto ld = [](const char *bp, unsigned off) -> float { return *(const float *)(bp + off); };
 
   /* this thread's halo cell */
   int hlx = 0, hly = 0, hcol = 0, hslot = -1;
@@ -404,18 +424,29 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     hcol = wrapi(y0 + hly, a.Ny) * a.P + wrapi(x0 + hlx, a.Nx);
   }
   const int hd = (hly + 1) * DW + (hlx + 1);           /* ring-1 slot (valid when hasR1) */
+  const unsigned colb = (unsigned)col * 4u, hcolb = (unsigned)hcol * 4u;
   const int od = (ty + 1) * DW + (tx + 1);             /* own slot in the ring-1 region  */
 
-  auto cv = [&](int f, int z, int c) -> float {       /* value with vertical boundary rules */
-    if (z >= 0 && z < Nz) return fin[f][z * ps + c];
-    if (f == 0) return prof(a, PR_RHO, z);
-    if (f == 4) return prof(a, PR_RTH, z);
-    if (z < 0) return fin[f][c];
-    if (f == 1) return a.uTop;
-    if (f == 2) return a.vTop;
-    if (f == 3) return 0.f;
-    if (C::TKE && f == C::IE) return 0.f;
-    return fin[f][(Nz - 1) * ps + c];
+  /* every field of level zz of the column at byte offset cb, with the vertical
+   * boundary rules applied for levels outside [0, Nz) (one uniform branch) */
+  auto lvl = [&](float (&v)[NF], int zz, unsigned cb) {
+    if (zz >= 0 && zz < Nz) {
+      const unsigned o = (unsigned)zz * pb + cb;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[f] = ld(fin[f], o);
+    } else {
+      const bool top = zz >= Nz;
+      const unsigned o = top ? (unsigned)(Nz - 1) * pb + cb : cb;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if (f == 0) v[f] = prof(a, PR_RHO, zz);
+        else if (f == 4) v[f] = prof(a, PR_RTH, zz);
+        else if (top && f == 1) v[f] = a.uTop;
+        else if (top && f == 2) v[f] = a.vTop;
+        else if (top && (f == 3 || (C::TKE && f == C::IE))) v[f] = 0.f;
+        else v[f] = ld(fin[f], o);
+      }
+    }
   };
   auto rthm_of = [&](float rth, float rho, float qv) {
     if constexpr (C::NM > 0) return rth * (1.f + a.Rv_Rg * qv * frcp(rho) * 1e-3f);
@@ -433,15 +464,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   /* register window of the own column, z-3..z+3 */
   float win[NF][7];
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
+  for (int j = 0; j < OD + 3; ++j) {
+    float t[NF];
+    lvl(t, j - 3, colb);
 #pragma unroll
-    for (int j = 0; j < 7; ++j) win[f][j] = cv(f, j - 3, col);
+    for (int f = 0; f < NF; ++f) win[f][j] = t[f];
+  }
   /* halo values of levels z .. z+PD */
   float hv[PD + 1][NF];
 #pragma unroll
-  for (int l = 0; l < PD; ++l)
+  for (int l = 0; l < PD; ++l) {
+    if (hasH) lvl(hv[l], l, hcolb);
+    else {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) hv[l][f] = hasH ? cv(f, l, hcol) : 0.f;
+      for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
+    }
+  }
 
   float pz[3];
   {
@@ -465,23 +503,26 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   if constexpr (PH) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) riz[j] = frcp(win[0][j + 1]);
-    kmo = a.km[1 * ps + col]; lso = a.ls[1 * ps + col];
-    kmo1 = a.km[2 * ps + col]; lso1 = a.ls[2 * ps + col];
-    if (hasR1) { kh = a.km[1 * ps + hcol]; lh = a.ls[1 * ps + hcol]; }
+    const char *kmb = (const char *)a.km, *lsb = (const char *)a.ls;
+    kmo = ld(kmb, pb + colb); lso = ld(lsb, pb + colb);
+    kmo1 = ld(kmb, 2 * pb + colb); lso1 = ld(lsb, 2 * pb + colb);
+    if (hasR1) { kh = ld(kmb, pb + hcolb); lh = ld(lsb, pb + hcolb); }
     /* u,v,w of levels -1 and 0, theta and moisture of level 0, on the ring-1 region */
     for (int i = tid; i < DW * DH; i += NT) {
       int dx = i % DW - 1, dy = i / DW - 1;
       int c = wrapi(y0 + dy, a.Ny) * a.P + wrapi(x0 + dx, a.Nx);
 #pragma unroll
       for (int lev = -1; lev <= 0; ++lev) {
-        float r = frcp(cv(0, lev, c));
-        VL(lev, 0)[dy + 1][dx + 1] = cv(1, lev, c) * r;
-        VL(lev, 1)[dy + 1][dx + 1] = cv(2, lev, c) * r;
-        VL(lev, 2)[dy + 1][dx + 1] = cv(3, lev, c) * r;
+        float t[NF];
+        lvl(t, lev, (unsigned)c * 4u);
+        float r = frcp(t[0]);
+        VL(lev, 0)[dy + 1][dx + 1] = t[1] * r;
+        VL(lev, 1)[dy + 1][dx + 1] = t[2] * r;
+        VL(lev, 2)[dy + 1][dx + 1] = t[3] * r;
         if (lev == 0) {
-          VL(lev, 3)[dy + 1][dx + 1] = cv(4, lev, c) * r;
+          VL(lev, 3)[dy + 1][dx + 1] = t[4] * r;
 #pragma unroll
-          for (int m = 0; m < C::NM; ++m) VL(lev, 4 + m)[dy + 1][dx + 1] = cv(C::IQV + m, lev, c) * r;
+          for (int m = 0; m < C::NM; ++m) VL(lev, 4 + m)[dy + 1][dx + 1] = t[C::IQV + m] * r;
         }
       }
     }
@@ -509,23 +550,34 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   }
 
   for (int z = 0; z < Nz; ++z) {
-    /* ---- prefetch: own column z+4, halo z+PD, RK base z; physics: Km/l ---- */
+    /* ---- loads, in order of first use (vector-memory waits are in order):
+     *      RK base of level z and, in the plain stage, own level z+3 (both used
+     *      in this level's P3); then the halo of level z+PD and the own level
+     *      z+4 of the physics stage and its Km/l (used in later levels) ---- */
     float nxt[NF], bse[NF];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) nxt[f] = cv(f, z + 4, col);
-    if (hasH) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) hv[PD][f] = cv(f, z + PD, hcol);
-    }
+    const unsigned oz = (unsigned)z * pb + colb;
     if (!a.emitF && !baseIn) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bse[f] = a.base[(long)f * a.fs + z * ps + col];
+      for (int f = 0; f < NF; ++f) bse[f] = ld(fbs[f], oz);
     }
+    if constexpr (OD == 3) {
+      lvl(nxt, z + 3, colb);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) win[f][6] = nxt[f];
+    }
+    if (hasH) lvl(hv[PD], z + PD, hcolb);
+    if constexpr (OD == 4) lvl(nxt, z + 4, colb);
     float kmo2 = 0.f, lso2 = 0.f, khn = 0.f, lhn = 0.f;
     if constexpr (PH) {
-      kmo2 = a.km[(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * ps + col];
-      lso2 = a.ls[(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * ps + col];
-      if (hasR1) { khn = a.km[(z + 2) * ps + hcol]; lhn = a.ls[(z + 2) * ps + hcol]; }
+      const char *kmb = (const char *)a.km, *lsb = (const char *)a.ls;
+      const unsigned o3 = (unsigned)(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * pb + colb;
+      kmo2 = ld(kmb, o3);
+      lso2 = ld(lsb, o3);
+      if (hasR1) {
+        const unsigned o2 = (unsigned)(z + 2) * pb + hcolb;
+        khn = ld(kmb, o2);
+        lhn = ld(lsb, o2);
+      }
     }
 
     /* ---- P1 ---- */
@@ -823,20 +875,19 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       }
 
       if (active) {
-        const int o = z * ps + col;
         if (a.emitF) {
 #pragma unroll
-          for (int f = 0; f < NF; ++f) a.out[(long)f * a.fs + o] = F[f];
+          for (int f = 0; f < NF; ++f) *(float *)(fout[f] + oz) = F[f];
         } else {
 #pragma unroll
           for (int f = 0; f < NF; ++f) {
             float v = fmaf(a.coefdt, F[f], baseIn ? win[f][3] : bse[f]);
             if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
             if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
-            a.out[(long)f * a.fs + o] = v;
+            *(float *)(fout[f] + oz) = v;
           }
         }
-        if (a.pout) a.pout[o] = pz[1];
+        if (a.pout) *(float *)((char *)a.pout + oz) = pz[1];
       }
 #pragma unroll
       for (int f = 0; f < NF; ++f) gzlo[f] = gzhi[f];
@@ -848,7 +899,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     for (int f = 0; f < NF; ++f) {
 #pragma unroll
       for (int j = 0; j < 6; ++j) win[f][j] = win[f][j + 1];
-      win[f][6] = nxt[f];
+      if constexpr (OD == 4) win[f][6] = nxt[f];
     }
 #pragma unroll
     for (int l = 0; l < PD; ++l)

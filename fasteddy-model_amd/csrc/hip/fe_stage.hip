@@ -74,10 +74,20 @@ __device__ __forceinline__ float weno3_side(float a, float b, float c) {
 template <int S>
 __device__ __forceinline__ float face_flux(float U, float omb, float m3, float m2, float m1, float c0, float p1, float p2) {
   if constexpr (S == 3) {        /* hybrid 5th/6th: b=0 is 5th-order upwind (WS2002) */
-    float cen = fmaf(37.f, c0 + m1, fmaf(-8.f, p1 + m2, p2 + m3));
-    float upw = fmaf(10.f, c0 - m1, fmaf(-5.f, p1 - m2, p2 - m3));
-    float sg = copysignf(omb, U);
-    return U * ((cen - sg * upw) * (1.f / 60.f));
+    /* 37(c0+m1) - 8(p1+m2) + (p2+m3) - s[10(c0-m1) - 5(p1-m2) + (p2-m3)], s = (1-b) sgn(U),
+     * as one 6-point stencil whose weights depend only on s: they are shared by every
+     * field advected through the face, leaving one multiply and five FMAs per field */
+    const float sg = copysignf(omb, U);
+    const float w0 = fmaf(-10.f, sg, 37.f), w1 = fmaf(10.f, sg, 37.f);
+    const float w2 = fmaf(5.f, sg, -8.f), w3 = fmaf(-5.f, sg, -8.f);
+    const float w4 = 1.f - sg, w5 = 1.f + sg;
+    float t = w0 * c0;
+    t = fmaf(w1, m1, t);
+    t = fmaf(w2, p1, t);
+    t = fmaf(w3, m2, t);
+    t = fmaf(w4, p2, t);
+    t = fmaf(w5, m3, t);
+    return (U * (1.f / 60.f)) * t;
   } else if constexpr (S == 2) { /* hybrid 3rd/4th */
     float cen = fmaf(7.f, c0 + m1, -(p1 + m2));
     float upw = fmaf(-3.f, c0 - m1, p1 - m2);
@@ -96,10 +106,15 @@ __device__ __forceinline__ float face_flux(float U, float omb, float m3, float m
   }
 }
 
-/* ---- perturbation pressure (cuda_pressureDevice.cu:53-113) ---- */
-__device__ __forceinline__ float pprime(const StepArgs &a, float rthm, int z) {
-  if (a.pform == 0) return powf(rthm * a.c1p, a.gam) - prof(a, PR_PBSRAW, z);
-  float r = (rthm - prof(a, PR_RTH, z)) * prof(a, PR_IRTH, z);
+/* ---- perturbation pressure (cuda_pressureDevice.cu:53-113) of rho*theta*(1 + mf), with
+ *      mf = (R_v/R) q_v the moisture factor (0 when dry).  r = rho*theta_m/(rho*theta)_BS - 1 is
+ *      formed from the small parts, (rho*theta - (rho*theta)_BS) and mf, so that it keeps
+ *      full relative precision even where moisture makes theta_m differ from the base state
+ *      by a few percent. ---- */
+__device__ __forceinline__ float pprime(const StepArgs &a, float rth, float mf, int z) {
+  if (a.pform == 0) return powf(rth * (1.f + mf) * a.c1p, a.gam) - prof(a, PR_PBSRAW, z);
+  const float ib = prof(a, PR_IRTH, z);
+  float r = fmaf(rth * ib, mf, (rth - prof(a, PR_RTH, z)) * ib);
   float s;
   if (fabsf(r) < 0.125f) {
     s = fmaf(r, a.pc[7], a.pc[6]);
@@ -448,9 +463,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       }
     }
   };
-  auto rthm_of = [&](float rth, float rho, float qv) {
-    if constexpr (C::NM > 0) return rth * (1.f + a.Rv_Rg * qv * frcp(rho) * 1e-3f);
-    else return rth;
+  auto mfac = [&](float rho, float qv) {                  /* moisture factor of theta_m */
+    if constexpr (C::NM > 0) return a.Rv_Rg * qv * frcp(rho) * 1e-3f;
+    else return 0.f;
   };
 
   float omb[NF];
@@ -484,9 +499,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   float pz[3];
   {
     float q3 = C::NM > 0 ? win[C::IQV][3] : 0.f, q4 = C::NM > 0 ? win[C::IQV][4] : 0.f;
-    pz[1] = pprime(a, rthm_of(win[4][3], win[0][3], q3), 0);
+    pz[1] = pprime(a, win[4][3], mfac(win[0][3], q3), 0);
     pz[0] = pz[1];
-    pz[2] = (Nz > 1) ? pprime(a, rthm_of(win[4][4], win[0][4], q4), 1) : pz[1];
+    pz[2] = (Nz > 1) ? pprime(a, win[4][4], mfac(win[0][4], q4), 1) : pz[1];
   }
   float gzlo[NF];
 #pragma unroll
@@ -592,7 +607,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     if (hasR1) {
       float q = C::NM > 0 ? hv[0][C::IQV] : 0.f;
       (&dvr[0][0])[hd] = frcp(hv[0][0]);
-      (&dvp[0][0])[hd] = pprime(a, rthm_of(hv[0][4], hv[0][0], q), z);
+      (&dvp[0][0])[hd] = pprime(a, hv[0][4], mfac(hv[0][0], q), z);
     }
     if constexpr (PH) {
       auto vb = [&](int q) -> Plane & { return VL(z + 1, q); };
@@ -614,7 +629,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
         for (int m = 0; m < C::NM; ++m) (&vb(4 + m)[0][0])[hd] = hv[1][C::IQV + m] * r;
       }
-      const float idl = 2.f / prof(a, PR_DELTA, z);
+      const float idl = 2.f * frcp(prof(a, PR_DELTA, z));
       {
         float A = win[0][3] * kmo;
         (&kmp[0][0][0])[od] = A;
@@ -631,6 +646,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     /* ---- P2: faces ---- */
     /* horizontal p' differences of the own cell, read here so that the p' plane
      * is free to be rewritten in the next level's first phase */
+    float t11 = 0.f, t21 = 0.f, t22 = 0.f;          /* own-cell SGS stresses, P2 -> P3 */
     const float pgx = 0.5f * a.dXi * (dvp[ty + 1][tx + 2] - dvp[ty + 1][tx]);
     const float pgy = 0.5f * a.dYi * (dvp[ty + 2][tx + 1] - dvp[ty][tx + 1]);
     {
@@ -693,8 +709,34 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
           for (int m = 0; m < C::NM; ++m) tb2[4 + m][fy][fx] = -Bv * a.dYi * (vz(4 + m)[Y][X] - vz(4 + m)[Y - 1][X]);
         };
-        xface(tx, ty, col);
-        yface(tx, ty, col);
+        /* own cell: the strain terms shared by its x and y faces once; tau_31 and tau_32 of
+         * the own column at level z are the z-face stresses carried from the previous level */
+        {
+          const int X = tx + 1, Y = ty + 1;
+          float A = kmp[0][Y][X], Bv = kmp[1][Y][X], E = C::TKE ? (2.f / 3.f) * raw[C::IE][ty + HH][tx + HH] : 0.f;
+          float s11 = a.dXi * (vz(0)[Y][X] - vz(0)[Y][X - 1]);
+          float s22 = a.dYi * (vz(1)[Y][X] - vz(1)[Y - 1][X]);
+          float s33 = J33z * a.dZi * (vz(2)[Y][X] - vm(2)[Y][X]);
+          float s21 = 0.25f * (a.dXi * ((vz(1)[Y][X] - vz(1)[Y][X - 1]) + (vz(1)[Y - 1][X] - vz(1)[Y - 1][X - 1]))
+                             + a.dYi * ((vz(0)[Y][X] - vz(0)[Y - 1][X]) + (vz(0)[Y][X - 1] - vz(0)[Y - 1][X - 1])));
+          const float iso = (2.f / 3.f) * A * (s11 + s22 + s33);
+          t11 = -2.f * A * s11 + iso + E;
+          t22 = -2.f * A * s22 + iso + E;
+          t21 = -2.f * A * s21;
+          tb1[0][ty][tx] = t11;
+          tb1[1][ty][tx] = t21;
+          tb1[2][ty][tx] = zc[0];
+          tb1[3][ty][tx] = -Bv * a.dXi * (vz(3)[Y][X] - vz(3)[Y][X - 1]);
+          tb2[0][ty][tx] = t21;
+          tb2[1][ty][tx] = t22;
+          tb2[2][ty][tx] = zc[1];
+          tb2[3][ty][tx] = -Bv * a.dYi * (vz(3)[Y][X] - vz(3)[Y - 1][X]);
+#pragma unroll
+          for (int m = 0; m < C::NM; ++m) {
+            tb1[4 + m][ty][tx] = -Bv * a.dXi * (vz(4 + m)[Y][X] - vz(4 + m)[Y][X - 1]);
+            tb2[4 + m][ty][tx] = -Bv * a.dYi * (vz(4 + m)[Y][X] - vz(4 + m)[Y - 1][X]);
+          }
+        }
         if (wave == 2) {
           yface(tx, TY, z == 0 ? wrapi(y0 + TY, a.Ny) * a.P + cx : 0);
         } else if (wave == 3 && tx < TY) {
@@ -705,8 +747,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           auto ecell = [&](int lx, int ly) { return raw[C::IE][ly + HH][lx + HH] * frcp(raw[0][ly + HH][lx + HH]); };
           auto qxf = [&](int lx, int ly) { return 0.5f * a.dXi * (ecell(lx + 1, ly) - ecell(lx - 1, ly)) * kmp[0][ly + 1][lx + 1]; };
           auto qyf = [&](int lx, int ly) { return 0.5f * a.dYi * (ecell(lx, ly + 1) - ecell(lx, ly - 1)) * kmp[0][ly + 1][lx + 1]; };
-          qx[ty][tx + 1] = qxf(tx, ty);
-          qy[ty + 1][tx] = qyf(tx, ty);
+          /* own cell: e of the four neighbours with rho^-1 from the ring-1 plane */
+          auto eown = [&](int lx, int ly) { return raw[C::IE][ly + HH][lx + HH] * dr[ly + 1][lx + 1]; };
+          qx[ty][tx + 1] = 0.5f * a.dXi * (eown(tx + 1, ty) - eown(tx - 1, ty)) * kmp[0][ty + 1][tx + 1];
+          qy[ty + 1][tx] = 0.5f * a.dYi * (eown(tx, ty + 1) - eown(tx, ty - 1)) * kmp[0][ty + 1][tx + 1];
           if (wave == 4) qy[0][tx] = qyf(tx, -1);
           else if (wave == 5) qy[TY + 1][tx] = qyf(tx, TY);
           else if (wave == 6 && tx < 2 * TY) {
@@ -778,7 +822,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float zn[4 + C::NM];
         {
           float A = kmo1 * win[0][4];
-          float Bv = A * (1.f + 2.f * lso1 / prof(a, PR_DELTA, z + 1));
+          float Bv = A * fmaf(2.f * lso1, frcp(prof(a, PR_DELTA, z + 1)), 1.f);
           float E = C::TKE ? (2.f / 3.f) * win[C::IE][4] : 0.f;
           float s33 = J33p * a.dZi * (vp(2)[Y][X] - vz(2)[Y][X]);
           float s11 = a.dXi * (vp(0)[Y][X] - vp(0)[Y][X - 1]);
@@ -808,10 +852,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           const float rhoe = win[C::IE][3];
           const float e = rhoe * rin;
           const float th = win[4][3] * rin;
-          float src = a.g / th * zc[3];
+          float src = a.g * frcp(th) * zc[3];
           {
-            float ce = 1.9f * a.c_k + (0.93f - 1.9f * a.c_k) * lso / prof(a, PR_DELTA, z);
-            src += -ce * rhoe * sqrtf(e) / lso;
+            const float ils = frcp(lso);
+            float ce = 1.9f * a.c_k + (0.93f - 1.9f * a.c_k) * lso * frcp(prof(a, PR_DELTA, z));
+            src += -ce * rhoe * sqrtf(e) * ils;
           }
           {
             float gxu = 0.5f * a.dXi * (vz(0)[Y][X + 1] - vz(0)[Y][X - 1]);
@@ -823,8 +868,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             float gxw = 0.5f * a.dXi * (vz(2)[Y][X + 1] - vz(2)[Y][X - 1]);
             float gyw = 0.5f * a.dYi * (vz(2)[Y + 1][X] - vz(2)[Y - 1][X]);
             float gzw = 0.5f * zdz * (vp(2)[Y][X] - vm(2)[Y][X]);
-            float t11 = tb1[0][ty][tx], t21 = tb1[1][ty][tx], t31 = tb1[2][ty][tx];
-            float t22 = tb2[1][ty][tx], t32 = tb2[2][ty][tx], t33 = zc[2];
+            const float t31 = zc[0], t32 = zc[1], t33 = zc[2];
             src -= t11 * gxu + t21 * gyu + t31 * gzu + t21 * gxv + t22 * gyv + t32 * gzv + t31 * gxw + t32 * gyw + t33 * gzw;
           }
           {
@@ -909,7 +953,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     pz[1] = pz[2];
     if (z + 2 < Nz) {
       float q = C::NM > 0 ? win[C::IQV][4] : 0.f;
-      pz[2] = pprime(a, rthm_of(win[4][4], win[0][4], q), z + 2);
+      pz[2] = pprime(a, win[4][4], mfac(win[0][4], q), z + 2);
     }
     if constexpr (PH) {
 #pragma unroll

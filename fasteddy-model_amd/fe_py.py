@@ -393,3 +393,21 @@ def turbulent_state(m_or_arrays, seed=1, amp_u=1.0, amp_th=0.5, e_range=(0.05, 0
     hydro[4][sl] = (hydro[4][sl] + rho * wave(amp_th)).astype(hydro.dtype)
     if tke is not None:
         tke[0][sl] = (rho * rng.uniform(e_range[0], e_range[1], size=(nx, ny, nz))).astype(tke.dtype)
+
+
+def moist_state(m, seed=1, nh=3, dq=1.0, ql_max=0.3):
+    """Set rho*q_v from the shallow-cumulus profile (17 g/kg at the ground, 13 g/kg at 1.5 km,
+    4 g/kg at 3 km) plus seeded uniform +/-dq g/kg noise, and rho*q_l = rho*U(0, ql_max) g/kg in
+    half of the cells, so that sub- and super-saturated, cloudy and clear cells all occur."""
+    rng = np.random.default_rng(seed)
+    hydro, mq, zpos = m.array("hydroFlds"), m.array("moistScalars"), m.array("zPos")
+    _, nxh, nyh, nzh = hydro.shape
+    sl = (slice(nh, nxh - nh), slice(nh, nyh - nh), slice(nh, nzh - nh))
+    z = zpos[sl].astype(np.float64)
+    qv = np.interp(z, [0.0, 1500.0, 3000.0, 1e6], [17.0, 13.0, 4.0, 4.0])
+    qv = qv + rng.uniform(-dq, dq, size=qv.shape)
+    rho = hydro[0][sl].astype(np.float64)
+    mq[0][sl] = (rho * np.maximum(qv, 0.0)).astype(mq.dtype)
+    if mq.shape[0] > 1:
+        ql = rng.uniform(0.0, ql_max, size=qv.shape) * (rng.uniform(size=qv.shape) < 0.5)
+        mq[1][sl] = (rho * ql).astype(mq.dtype)

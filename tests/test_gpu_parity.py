@@ -136,3 +136,89 @@ def test_rk3_steps_match_oracle(tmp_path):
         assert m.geti("simTime_it") == 10
     finally:
         m.close()
+
+
+def make_moist(params, tmp_path, seed):
+    m = fe_py.Model(params, str(tmp_path))
+    fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=seed)
+    fe_py.moist_state(m, seed=seed + 1)
+    m.upload()
+    o32, o64 = fe_py.Oracle(params, 32), fe_py.Oracle(params, 64)
+    o32.load_from_model(m)
+    o64.load_from_model(m)
+    return m, o32, o64
+
+
+@pytest.mark.parametrize("shape,dz", [((24, 16, 20), 25.0), ((40, 10, 12), 120.0)])
+def test_stage_tendencies_moist(shape, dz, tmp_path):
+    """Shallow-cumulus physics (q_v, q_l with hybrid 5th and WENO5 advection, SGS moisture
+    fluxes, saturation adjustment, moist buoyancy and pressure): tendencies of all three
+    stages; the second grid reaches 1.4 km, where the profile is supersaturated."""
+    nx, ny, nz = shape
+    p = dict(fe_py.CFG2, Nx=nx, Ny=ny, Nz=nz, d_zeta=dz)
+    m, o32, o64 = make_moist(p, tmp_path, seed=nx)
+    try:
+        assert "moist2" in m.L.fe_kernel_info().decode()
+        fc_any = False
+        for stage in (0, 1, 2):
+            check_tendencies(m, o32, o64, stage, f"moist {shape}")
+        fc = m.interior(o64.arr("FCOND").reshape(o64.arr("HYDRO").shape[1:]))
+        fc_any = np.any(fc > 0) and np.any(fc < 0) and np.any(fc == 0)
+        assert fc_any, "expected condensation, evaporation and inactive cells"
+    finally:
+        m.close()
+
+
+def test_rk3_steps_moist(tmp_path):
+    """Three full moist RK3 steps (microphysics every step) against the FP64 oracle."""
+    p = dict(fe_py.CFG2, Nx=32, Ny=16, Nz=16, d_zeta=80.0, NtBatch=1)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=3)
+        fe_py.moist_state(m, seed=4)
+        o64 = fe_py.Oracle(p, 64)
+        o64.load_from_model(m)
+        h0 = m.interior(m.array("hydroFlds")).astype(np.float64).copy()
+        q0 = m.interior(m.array("moistScalars")).astype(np.float64).copy()
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        for _ in range(3):
+            m.commence()
+        o64.steps(3)
+        for name, oname, ref0 in (("hydroFlds", "HYDRO", h0), ("moistScalars", "MOIST", q0)):
+            g = m.interior(m.array(name)).astype(np.float64)
+            o = m.interior(o64.arr(oname)[: g.shape[0]])
+            change = np.abs(o - ref0).max(axis=(1, 2, 3))
+            diff = np.abs(g - o).max(axis=(1, 2, 3))
+            print(f"moist 3 steps {name}: |gpu-fp64| {diff} vs |change| {change}")
+            assert np.all(diff <= 2e-3 * change + 1e-6 * np.abs(o).max(axis=(1, 2, 3))), (name, diff, change)
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("name", ["dry_12x10x12", "moist_10x8x10"])
+def test_stage_tendencies_against_golden(name, tmp_path):
+    """The committed oracle fixtures: load the stored state, build the stage-0 and stage-2
+    tendencies on the GPU and compare with the stored FP64/FP32 oracle tendencies."""
+    import json
+    import os
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"{name}.npz"))
+    params = json.loads(str(ref["params"]))
+    m = fe_py.Model(params, str(tmp_path))
+    try:
+        for host in ("hydroFlds", "sgstkeScalars", "moistScalars"):
+            if host in ref.files:
+                m.array(host)[...] = ref[host]
+        m.upload()
+        for stage in (0, 2):
+            m.build_frhs(stage)
+            for fhost in ("hydroFldsFrhs", "sgstkeScalarsFrhs", "moistScalarsFrhs"):
+                k64 = f"{fhost}_s{stage}_f64"
+                if k64 not in ref.files:
+                    continue
+                g = m.interior(m.array(fhost)).astype(np.float64)
+                r64, r32 = ref[k64], ref[f"{fhost}_s{stage}_f32"].astype(np.float64)
+                e_gpu, e_ref = rel_err(g, r64), rel_err(r32, r64)
+                print(f"golden {name} stage {stage} {fhost}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
+                assert np.all(e_gpu <= np.maximum(1e-5, 2 * e_ref)), (fhost, e_gpu, e_ref)
+    finally:
+        m.close()

@@ -121,7 +121,7 @@ def main():
         "traffic": traffic_from_profiles("stage_kernel"),
         "kernel": "fe::stage_kernel (fused tendency + RK3 stage update, 3 launches/step)",
         "avg_launch_ms": {"stage_plain": ms[1] / max(n_plain, 1), "stage_physics": ms[2] / max(n_phys, 1),
-                          "column_pass": ms[0] / max(cnt[0], 1)},
+                          "surface_pass": ms[0] / max(cnt[0], 1)},
         "step_frac": value * 32 * nprog / PEAK_HBM,
         "copy_GBps": copy_bw,
     }

@@ -10,9 +10,9 @@
  *
  * One RK3 step is three fused stage launches with out-of-place buffers:
  *   stage 1: S0 -> S1, stage 2: (S1, S0) -> S2, stage 3: (S2, S0) -> S0,
- * the physics stage preceded by a column pass (SGS length scale, eddy
- * viscosity, surface layer).  All launches go to one non-blocking stream;
- * the host synchronises once per batch.
+ * the physics stage preceded by a surface-layer pass over the lowest level.
+ * All launches go to one non-blocking stream; the host synchronises once per
+ * batch.
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -53,7 +53,7 @@ struct Dev {
   std::vector<float> hprof;
   float *prof = nullptr;
   float *S[3] = {nullptr, nullptr, nullptr};
-  float *ls = nullptr, *km = nullptr, *pout = nullptr;
+  float *pout = nullptr;
   float *surf = nullptr;           /* 15 planes of [Ny][P] */
   float *stg = nullptr;            /* halo-inclusive staging, one 3-D field */
   long n3h = 0, n2h = 0;
@@ -61,7 +61,7 @@ struct Dev {
   float ms_per_step = 0;
   int gridBlocks[3] = {0, 0, 0};
   int tb[3] = {0, 0, 0};
-  /* per-kernel HIP-event timing on the launch stream: class 0 column pass, 1 plain stage, 2 physics stage */
+  /* per-kernel HIP-event timing on the launch stream: class 0 surface-layer pass, 1 plain stage, 2 physics stage */
   int timing = 0;
   std::vector<hipEvent_t> ev;
   std::vector<int> evcls;
@@ -133,11 +133,11 @@ __global__ void k_2d_to_host(const float *src, float *dst, int Nx, int Ny, int N
 /* --------------------------------------------------------------------------- */
 /* kernel variant table                                                         */
 typedef void (*stage_fn)(StepArgs);
-typedef void (*column_fn)(StepArgs, int, int);
+typedef void (*surface_fn)(StepArgs);
 
 struct Variant {
   stage_fn plain, phys;
-  column_fn column;
+  surface_fn surface;
   const char *name;
 };
 
@@ -146,7 +146,7 @@ Variant make_variant(const char *name) {
   Variant v;
   v.plain = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>>;
   v.phys = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
-  v.column = column_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
+  v.surface = surface_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
   v.name = name;
   return v;
 }
@@ -178,7 +178,7 @@ int select_variant(void) {
 
 void fill_args(StepArgs &a, int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
   memset(&a, 0, sizeof a);
-  a.in = in; a.base = base; a.out = out; a.prof = D.prof; a.ls = D.ls; a.km = D.km; a.pout = pout;
+  a.in = in; a.base = base; a.out = out; a.prof = D.prof; a.pout = pout;
   a.s.cd = surfp(SF_CD); a.s.ch = surfp(SF_CH); a.s.cq = surfp(SF_CQ); a.s.ust = surfp(SF_UST); a.s.hf = surfp(SF_HF);
   a.s.tsk = surfp(SF_TSK); a.s.iol = surfp(SF_IOL); a.s.z0m = surfp(SF_Z0M); a.s.z0t = surfp(SF_Z0T); a.s.qf = surfp(SF_QF);
   a.s.qsk = surfp(SF_QSK); a.s.t13 = surfp(SF_T13); a.s.t23 = surfp(SF_T23); a.s.th3 = surfp(SF_TH3); a.s.tq3 = surfp(SF_TQ3);
@@ -214,15 +214,15 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
 
 inline int physics_stage(int stage) { return (physics_oneRKonly == 0) || (stage == numRKstages); }
 
-/* launch one RK stage (column pass first when the stage carries the SGS physics) */
+/* launch one RK stage (surface-layer pass first when the stage carries the physics) */
 int launch_stage(int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
   StepArgs a;
   fill_args(a, stage, in, base, out, emitF, pout);
   const int phys = physics_stage(stage) && turbulenceSelector > 0;
-  if (physics_stage(stage) && (phys || surflayerSelector > 0)) {
+  if (physics_stage(stage) && surflayerSelector > 0) {
     dim3 cg((D.Nx + 255) / 256, D.Ny);
     timed_begin(0);
-    hipLaunchKernelGGL(g_var.column, cg, dim3(256), 0, D.stream, a, phys, surflayerSelector > 0);
+    hipLaunchKernelGGL(g_var.surface, cg, dim3(256), 0, D.stream, a);
     timed_end();
   }
   dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY);
@@ -363,7 +363,7 @@ int fecuda_Cleanup(void) {
 }
 
 int fecuda_SetBlocksPerGrid(int Nx_, int Ny_, int Nz_) {
-  /* recorded for reporting; the tile shape of the fused kernels is fixed at 64x4 columns */
+  /* recorded for reporting; the tile shape of the fused kernels is fixed at 64x8 columns */
   D.gridBlocks[0] = Nx_; D.gridBlocks[1] = Ny_; D.gridBlocks[2] = Nz_;
   return FE_OK;
 }
@@ -380,9 +380,8 @@ int cuda_gridInit(void) {
   D.ps = (long)D.Ny * D.P;
   D.fs = D.ps * D.Nz;
   D.np = Nzp + 10;
-  /* the kernels address within one field (and the Nz+2-level SGS arrays) with
-   * 32-bit byte offsets */
-  if ((D.Nz + 2) * D.ps * (long)sizeof(float) > 0xFFFFFFFFL) {
+  /* the kernels address within one field with 32-bit byte offsets */
+  if (D.Nz * D.ps * (long)sizeof(float) > 0xFFFFFFFFL) {
     fprintf(stderr, "cuda_gridInit: %d x %d x %d exceeds 4 GiB per field\n", Nxp, Nyp, Nzp);
     return FE_ERR_UNSUPPORTED;
   }
@@ -484,11 +483,6 @@ int cuda_hydroCoreInit(void) {
     HIPCHK(hipMalloc(&D.S[s], (size_t)D.NF * D.fs * sizeof(float)));
     HIPCHK(hipMemset(D.S[s], 0, (size_t)D.NF * D.fs * sizeof(float)));
   }
-  const size_t nlz = (size_t)(D.Nz + 2) * D.ps;
-  HIPCHK(hipMalloc(&D.ls, nlz * sizeof(float)));
-  HIPCHK(hipMalloc(&D.km, nlz * sizeof(float)));
-  HIPCHK(hipMemset(D.ls, 0, nlz * sizeof(float)));
-  HIPCHK(hipMemset(D.km, 0, nlz * sizeof(float)));
   HIPCHK(hipMalloc(&D.pout, (size_t)D.fs * sizeof(float)));
   HIPCHK(hipMemset(D.pout, 0, (size_t)D.fs * sizeof(float)));
   HIPCHK(hipMalloc(&D.surf, (size_t)SF_N * D.ps * sizeof(float)));
@@ -500,7 +494,7 @@ int cuda_hydroCoreInit(void) {
 
 int cuda_hydroCoreCleanup(void) {
   for (int s = 0; s < 3; ++s) if (D.S[s]) { hipFree(D.S[s]); D.S[s] = nullptr; }
-  float **p[] = {&D.ls, &D.km, &D.pout, &D.surf, &D.stg};
+  float **p[] = {&D.pout, &D.surf, &D.stg};
   for (auto q : p) if (*q) { hipFree(*q); *q = nullptr; }
   D.hydro = 0;
   return FE_OK;

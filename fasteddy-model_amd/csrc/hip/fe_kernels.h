@@ -41,8 +41,6 @@ struct StepArgs {
   const float *base;    /* phi^n                                    */
   float *out;           /* stage output (or F in emit mode)         */
   const float *prof;    /* [NPROF][Nz+6]                            */
-  float *ls;            /* SGS length scale  [Nz+2][Ny][P], z' = z+1 */
-  float *km;            /* eddy viscosity    [Nz+2][Ny][P]           */
   float *pout;          /* optional perturbation-pressure emission [Nz][Ny][P] */
   SurfArgs s;
   int Nx, Ny, Nz, P;

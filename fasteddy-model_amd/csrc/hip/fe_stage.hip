@@ -290,47 +290,19 @@ __device__ void surface_column(const StepArgs &a, long c2, float rho, float ru, 
   if (moist) { s.cq[c2] = cqi; s.qf[c2] = qf; s.qsk[c2] = qsk; s.tq3[c2] = qf * rho; }
 }
 
-/* Column pre-pass of the physics stage: SGS length scale (cuda_sgstkeDevice.cu:96-151)
- * and Lilly eddy viscosity Km(z) = c_k/2 (l sqrt(e)|z + l sqrt(e)|z-1)
- * (cuda_sgsTurbDevice.cu:202-254) for z in [-1, Nz], plus the surface layer.
- * Km(-1) = 0 as in the reference, which never computes it. */
+/* Surface-layer pre-pass of the physics stage (cuda_surfaceLayerDevice.cu:112-521):
+ * one thread per column at the lowest level; writes the surface stresses and
+ * fluxes the stage kernel applies at z = 0, and the 2-D surface state. */
 template <class C>
-__global__ void __launch_bounds__(256) column_kernel(StepArgs a, int do_sgs, int do_surf) {
+__global__ void __launch_bounds__(256) surface_kernel(StepArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= a.Nx || y >= a.Ny) return;
   const long col = (long)y * a.P + x;
-  if (do_sgs && C::TKE) {
-    /* theta and e at z-1, z, z+1 with boundary rules */
-    auto th = [&](int z) { return colval<C>(a, 4, z, col) / colval<C>(a, 0, z, col); };
-    float thm = th(-2), th0 = th(-1), thp;
-    float qprev = 0.f;
-    for (int z = -1; z <= a.Nz; ++z) {
-      thp = th(z + 1);
-      float ri = 1.f / colval<C>(a, 0, z, col);
-      float e = colval<C>(a, C::IE, z, col) * ri;
-      float dthdz = 0.5f * prof(a, PR_J33, z) * a.dZi * (thp - thm);
-      float N2 = (a.g / th0) * dthdz;
-      float delta = prof(a, PR_DELTA, z);
-      float l = delta;
-      float se = sqrtf(e);
-      if (N2 > 0.f) l = fmaxf(fminf(0.76f * se * rsqrtf(N2), delta), 1e-2f);
-      float q = l * se;
-      float km = (z >= 0) ? a.c_k * 0.5f * (q + qprev) : 0.f;
-      long o = (long)(z + 1) * a.ps + col;
-      a.ls[o] = l;
-      a.km[o] = km;
-      qprev = q;
-      thm = th0;
-      th0 = thp;
-    }
-  }
-  if (do_surf) {
-    float z1 = 0.5f / (a.dZi * prof(a, PR_J33, 0));
-    float rho = a.in[col], ru = a.in[a.fs + col], rv = a.in[2 * a.fs + col], rth = a.in[4 * a.fs + col];
-    float rqv = C::NM > 0 ? a.in[C::IQV * a.fs + col] : 0.f;
-    surface_column(a, col, rho, ru, rv, rth, rqv, C::NM > 0, z1);
-  }
+  float z1 = 0.5f / (a.dZi * prof(a, PR_J33, 0));
+  float rho = a.in[col], ru = a.in[a.fs + col], rv = a.in[2 * a.fs + col], rth = a.in[4 * a.fs + col];
+  float rqv = C::NM > 0 ? a.in[C::IQV * a.fs + col] : 0.f;
+  surface_column(a, col, rho, ru, rv, rth, rqv, C::NM > 0, z1);
 }
 
 /* =============================================================================== */
@@ -356,6 +328,21 @@ __device__ __forceinline__ void halo_cell(int h, int &lx, int &ly) {
 #endif
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+/* SGS length scale (cuda_sgstkeDevice.cu:96-151) at level z from theta at z-1, z, z+1 and
+ * e = rho*e/rho at z: N^2 = g/theta dtheta/dz (centred); l = Delta, or
+ * max(min(0.76 sqrt(e)/N, Delta), 0.01) where N^2 > 0.  Returns l and q = l sqrt(e);
+ * the Lilly eddy viscosity is Km(z) = c_k/2 (q(z) + q(z-1)) (cuda_sgsTurbDevice.cu:202-254). */
+__device__ __forceinline__ void sgs_lq(const StepArgs &a, int z, float thm, float th0, float thp, float e,
+                                       float &l, float &q) {
+  const float dthdz = 0.5f * prof(a, PR_J33, z) * a.dZi * (thp - thm);
+  const float N2 = a.g * frcp(th0) * dthdz;
+  const float delta = prof(a, PR_DELTA, z);
+  const float se = sqrtf(e);
+  l = delta;
+  if (N2 > 0.f) l = fmaxf(fminf(0.76f * se * rsqrtf(N2), delta), 1e-2f);
+  q = l * se;
+}
 
 /* One fused RK stage.  Per level z, with two work-group barriers:
  *   P1  write the xy plane of level z (own cell from the register window,
@@ -514,14 +501,34 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   float riz[5];                         /* rho^-1 own column at z-2..z+2    */
   float kmo = 0.f, lso = 0.f;           /* own Km, l at level z             */
   float kmo1 = 0.f, lso1 = 0.f;         /* own Km, l at level z+1           */
-  float kh = 0.f, lh = 0.f;             /* ring-1 halo Km, l at level z     */
+  float qo1 = 0.f;                      /* own l sqrt(e) at level z+1       */
+  float kh = 0.f, lh = 0.f, qh = 0.f;   /* ring-1 cell: Km, l, l sqrt(e) at level z */
   if constexpr (PH) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) riz[j] = frcp(win[0][j + 1]);
-    const char *kmb = (const char *)a.km, *lsb = (const char *)a.ls;
-    kmo = ld(kmb, pb + colb); lso = ld(lsb, pb + colb);
-    kmo1 = ld(kmb, 2 * pb + colb); lso1 = ld(lsb, 2 * pb + colb);
-    if (hasR1) { kh = ld(kmb, pb + hcolb); lh = ld(lsb, pb + hcolb); }
+    /* l and Km of levels 0 and 1 (own column) and of level 0 (ring-1 cell); Km(0)
+     * takes l sqrt(e) of level -1, evaluated with the boundary values */
+    if constexpr (C::TKE) {
+      float th[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) th[j] = win[4][j + 1] * riz[j];     /* levels -2..2 */
+      float lm, qm, l0, q0, l1, q1;
+      sgs_lq(a, -1, th[0], th[1], th[2], win[C::IE][2] * riz[1], lm, qm);
+      sgs_lq(a, 0, th[1], th[2], th[3], win[C::IE][3] * riz[2], l0, q0);
+      sgs_lq(a, 1, th[2], th[3], th[4], win[C::IE][4] * riz[3], l1, q1);
+      kmo = a.c_k * 0.5f * (q0 + qm); lso = l0;
+      kmo1 = a.c_k * 0.5f * (q1 + q0); lso1 = l1; qo1 = q1;
+      if (hasR1) {
+        float t2[NF], t1[NF];
+        lvl(t2, -2, hcolb);
+        lvl(t1, -1, hcolb);
+        const float r2 = frcp(t2[0]), r1 = frcp(t1[0]), r0 = frcp(hv[0][0]), rp = frcp(hv[1][0]);
+        float lq, qq, l0h, q0h;
+        sgs_lq(a, -1, t2[4] * r2, t1[4] * r1, hv[0][4] * r0, t1[C::IE] * r1, lq, qq);
+        sgs_lq(a, 0, t1[4] * r1, hv[0][4] * r0, hv[1][4] * rp, hv[0][C::IE] * r0, l0h, q0h);
+        kh = a.c_k * 0.5f * (q0h + qq); lh = l0h; qh = q0h;
+      }
+    }
     /* u,v,w of levels -1 and 0, theta and moisture of level 0, on the ring-1 region */
     for (int i = tid; i < DW * DH; i += NT) {
       int dx = i % DW - 1, dy = i / DW - 1;
@@ -568,7 +575,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     /* ---- loads, in order of first use (vector-memory waits are in order):
      *      RK base of level z and, in the plain stage, own level z+3 (both used
      *      in this level's P3); then the halo of level z+PD and the own level
-     *      z+4 of the physics stage and its Km/l (used in later levels) ---- */
+     *      z+4 of the physics stage (used in later levels) ---- */
     float nxt[NF], bse[NF];
     const unsigned oz = (unsigned)z * pb + colb;
     if (!a.emitF && !baseIn) {
@@ -582,18 +589,6 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     }
     if (hasH) lvl(hv[PD], z + PD, hcolb);
     if constexpr (OD == 4) lvl(nxt, z + 4, colb);
-    float kmo2 = 0.f, lso2 = 0.f, khn = 0.f, lhn = 0.f;
-    if constexpr (PH) {
-      const char *kmb = (const char *)a.km, *lsb = (const char *)a.ls;
-      const unsigned o3 = (unsigned)(z + 3 < Nz + 2 ? z + 3 : Nz + 1) * pb + colb;
-      kmo2 = ld(kmb, o3);
-      lso2 = ld(lsb, o3);
-      if (hasR1) {
-        const unsigned o2 = (unsigned)(z + 2) * pb + hcolb;
-        khn = ld(kmb, o2);
-        lhn = ld(lsb, o2);
-      }
-    }
 
     /* ---- P1 ---- */
 #pragma unroll
@@ -938,6 +933,27 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       rin = rinp;
     }
 
+    /* ---- SGS length scale and eddy viscosity one level ahead (loads of this level are
+     *      consumed here, at the end, so that their latency is hidden) ---- */
+    float r3 = 0.f;
+    if constexpr (PH) {
+      r3 = frcp(win[0][6]);                             /* rho^-1 at z+3 */
+      if constexpr (C::TKE) {
+        /* own column: l, Km of level z+2 (theta at z+1..z+3, e at z+2) */
+        float l2, q2;
+        sgs_lq(a, z + 2, win[4][4] * riz[3], win[4][5] * riz[4], win[4][6] * r3, win[C::IE][5] * riz[4], l2, q2);
+        kmo = kmo1; lso = lso1;
+        kmo1 = a.c_k * 0.5f * (q2 + qo1); lso1 = l2; qo1 = q2;
+        /* ring-1 cell: l, Km of level z+1 (halo levels z..z+2) */
+        if (hasR1) {
+          const float h0 = frcp(hv[0][0]), h1 = frcp(hv[1][0]), h2 = frcp(hv[2][0]);
+          float l1h, q1h;
+          sgs_lq(a, z + 1, hv[0][4] * h0, hv[1][4] * h1, hv[2][4] * h2, hv[1][C::IE] * h1, l1h, q1h);
+          kh = a.c_k * 0.5f * (q1h + qh); lh = l1h; qh = q1h;
+        }
+      }
+    }
+
     /* ---- advance the pipelines ---- */
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -949,18 +965,16 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     for (int l = 0; l < PD; ++l)
 #pragma unroll
       for (int f = 0; f < NF; ++f) hv[l][f] = hv[l + 1][f];
+    if constexpr (PH) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) riz[j] = riz[j + 1];
+      riz[4] = r3;
+    }
     pz[0] = pz[1];
     pz[1] = pz[2];
     if (z + 2 < Nz) {
       float q = C::NM > 0 ? win[C::IQV][4] : 0.f;
       pz[2] = pprime(a, win[4][4], mfac(win[0][4], q), z + 2);
-    }
-    if constexpr (PH) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) riz[j] = riz[j + 1];
-      riz[4] = frcp(win[0][5]);
-      kmo = kmo1; lso = lso1; kmo1 = kmo2; lso1 = lso2;
-      kh = khn; lh = lhn;
     }
   }
 }

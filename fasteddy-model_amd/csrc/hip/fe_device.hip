@@ -144,7 +144,7 @@ struct Variant {
 template <bool TKE, int NM, int SH, int SE, int SQV, int SQL>
 Variant make_variant(const char *name) {
   Variant v;
-  v.plain = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>>;
+  v.plain = stage_plain_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>>;
   v.phys = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
   v.surface = surface_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
   v.name = name;
@@ -616,6 +616,79 @@ double fe_copy_probe(long nfloats, int reps) {
   hipEventElapsedTime(&ms, e0, e1);
   hipEventDestroy(e0); hipEventDestroy(e1);
   return (double)2 * nfloats * sizeof(float) * reps / (ms * 1e-3) / 1e9;
+}
+
+/* Horizontal-mean profile statistics of the device state in FP64, the on-device
+ * counterpart of the reference's post-processing of an output state
+ * (tutorials/notebooks/fetutorialfunctions.py:55-83,344-431).  One work-group per
+ * level accumulates the level sums of u, v, w, theta (rho-weighted fields over
+ * rho), their products, and the SGS TKE and moisture per unit mass; each
+ * wavefront reduces its lanes with cross-lane shuffles, and the wavefronts'
+ * partial sums meet in LDS.  Output columns, per level:
+ *   0 <u> 1 <v> 2 <w> 3 <theta> 4 <u'u'> 5 <u'w'> 6 <v'v'> 7 <v'w'> 8 <w'w'>
+ *   9 resolved TKE 10 <theta'w'> 11 <e> 12 <q_v> 13 <q_l>
+ * (covariances from FP64 first and second moments). */
+constexpr int PROF_NSTAT = 14, PROF_NSUM = 13, PROF_NT = 256;
+__global__ void __launch_bounds__(PROF_NT) k_profiles(const float *S, long fs, long ps, int Nx, int Ny, int P, int nTKE,
+                                                      int nM, double *out) {
+  const int z = blockIdx.x;
+  double acc[PROF_NSUM];
+#pragma unroll
+  for (int k = 0; k < PROF_NSUM; ++k) acc[k] = 0.0;
+  const int n = Nx * Ny;
+  for (int i = threadIdx.x; i < n; i += PROF_NT) {
+    const int y = i / Nx, x = i - y * Nx;
+    const long o = (long)z * ps + (long)y * P + x;
+    const double rho = S[o];
+    const double u = S[fs + o] / rho, v = S[2 * fs + o] / rho, w = S[3 * fs + o] / rho, th = S[4 * fs + o] / rho;
+    acc[0] += u; acc[1] += v; acc[2] += w; acc[3] += th;
+    acc[4] += u * u; acc[5] += u * w; acc[6] += v * v; acc[7] += v * w; acc[8] += w * w; acc[9] += th * w;
+    if (nTKE > 0) acc[10] += S[5 * fs + o] / rho;
+    if (nM > 0) acc[11] += S[(long)(5 + nTKE) * fs + o] / rho;
+    if (nM > 1) acc[12] += S[(long)(6 + nTKE) * fs + o] / rho;
+  }
+#pragma unroll
+  for (int k = 0; k < PROF_NSUM; ++k)
+    for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off);
+  __shared__ double part[PROF_NT / 64][PROF_NSUM];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < PROF_NSUM; ++k) part[wave][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s[PROF_NSUM];
+    for (int k = 0; k < PROF_NSUM; ++k) {
+      s[k] = 0.0;
+      for (int wv = 0; wv < PROF_NT / 64; ++wv) s[k] += part[wv][k];
+      s[k] /= (double)n;
+    }
+    double *r = out + (long)z * PROF_NSTAT;
+    r[0] = s[0]; r[1] = s[1]; r[2] = s[2]; r[3] = s[3];
+    r[4] = s[4] - s[0] * s[0];
+    r[5] = s[5] - s[0] * s[2];
+    r[6] = s[6] - s[1] * s[1];
+    r[7] = s[7] - s[1] * s[2];
+    r[8] = s[8] - s[2] * s[2];
+    r[9] = 0.5 * (r[4] + r[6] + r[8]);
+    r[10] = s[9] - s[3] * s[2];
+    r[11] = s[10]; r[12] = s[11]; r[13] = s[12];
+  }
+}
+
+int fe_profiles(double *out, int nstat) {
+  if (!D.hydro) return FE_ERR_STATE;
+  if (nstat != PROF_NSTAT || !out) return FE_ERR_UNSUPPORTED;
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, (size_t)D.Nz * PROF_NSTAT * sizeof(double)));
+  hipLaunchKernelGGL(k_profiles, dim3(D.Nz), dim3(PROF_NT), 0, D.stream, D.S[0], D.fs, D.ps, D.Nx, D.Ny, D.P, D.nTKE, D.nM, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)D.Nz * PROF_NSTAT * sizeof(double), hipMemcpyDeviceToHost, D.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
+  hipFree(d);
+  HIPCHK(e);
+  return FE_OK;
 }
 
 int fe_debug_build_frhs(int stage) {

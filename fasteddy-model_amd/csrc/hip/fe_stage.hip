@@ -160,29 +160,50 @@ struct Cfg {
   }
 };
 
-template <class C, int F>
+template <class C, int F, int FE = C::NF>
 struct FaceLoop {
   __device__ __forceinline__ static void xf(float (*raw)[RH][RW], float (*gx)[TY][TX + 1], int fy, int fx, float U, const float *omb) {
-    if constexpr (F < C::NF) {
+    if constexpr (F < FE) {
       const float *r = &raw[F][fy + HH][fx];
       gx[F][fy][fx] = face_flux<C::template sch<F>()>(U, omb[F], r[0], r[1], r[2], r[3], r[4], r[5]);
-      FaceLoop<C, F + 1>::xf(raw, gx, fy, fx, U, omb);
+      FaceLoop<C, F + 1, FE>::xf(raw, gx, fy, fx, U, omb);
     }
   }
   __device__ __forceinline__ static void yf(float (*raw)[RH][RW], float (*gy)[TY + 1][TX], int fy, int fx, float V, const float *omb) {
-    if constexpr (F < C::NF) {
+    if constexpr (F < FE) {
       gy[F][fy][fx] = face_flux<C::template sch<F>()>(V, omb[F], raw[F][fy][fx + HH], raw[F][fy + 1][fx + HH], raw[F][fy + 2][fx + HH],
                                                       raw[F][fy + 3][fx + HH], raw[F][fy + 4][fx + HH], raw[F][fy + 5][fx + HH]);
-      FaceLoop<C, F + 1>::yf(raw, gy, fy, fx, V, omb);
+      FaceLoop<C, F + 1, FE>::yf(raw, gy, fy, fx, V, omb);
     }
   }
-  __device__ __forceinline__ static void zf(float (*win)[7], float *gz, float W, const float *omb) {
-    if constexpr (F < C::NF) {
+  template <int WD>
+  __device__ __forceinline__ static void zf(float (*win)[WD], float *gz, float W, const float *omb) {
+    if constexpr (F < FE) {
       gz[F] = face_flux<C::template sch<F>()>(W, omb[F], win[F][1], win[F][2], win[F][3], win[F][4], win[F][5], win[F][6]);
-      FaceLoop<C, F + 1>::zf(win, gz, W, omb);
+      FaceLoop<C, F + 1, FE>::template zf<WD>(win, gz, W, omb);
     }
   }
 };
+
+/* Advective fluxes of fields [F0, F1) through the tile's ring faces: the y face
+ * row y = TY (one face per lane) and the x face column x = TX (lanes < TY).  The
+ * ring is split by field over several waves so that no wave carries a whole
+ * extra face pass. */
+template <class C, int F0, int F1>
+__device__ __forceinline__ void ring_faces_y(float (*raw)[RH][RW], float (*gy)[TY + 1][TX], const float (*dr)[DW], int tx,
+                                             float hyD, const float *omb) {
+  const int fx = tx, fy = TY;
+  const float V = hyD * (raw[2][fy + HH - 1][fx + HH] * dr[fy][fx + 1] + raw[2][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
+  FaceLoop<C, F0, F1>::yf(raw, gy, fy, fx, V, omb);
+}
+template <class C, int F0, int F1>
+__device__ __forceinline__ void ring_faces_x(float (*raw)[RH][RW], float (*gx)[TY][TX + 1], const float (*dr)[DW], int tx,
+                                             float hxD, const float *omb) {
+  if (tx >= TY) return;
+  const int fx = TX, fy = tx;
+  const float U = hxD * (raw[1][fy + HH][fx + HH - 1] * dr[fy + 1][fx] + raw[1][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
+  FaceLoop<C, F0, F1>::xf(raw, gx, fy, fx, U, omb);
+}
 
 /* SGS stress pieces on a face-located cell (cuda_sgsTurbDevice.cu:60-367).  All
  * arguments are velocities (momentum * rho^-1) around the staggered location. */
@@ -305,6 +326,85 @@ __global__ void __launch_bounds__(256) surface_kernel(StepArgs a) {
   surface_column(a, col, rho, ru, rv, rth, rqv, C::NM > 0, z1);
 }
 
+/* Dynamics part of the own-cell tendency at level z (both stage kernels):
+ * flux-form advection from the x/y face-flux differences dxy and the z face
+ * fluxes (the upper one computed here from the register window, the lower one
+ * carried), pressure gradient (cuda_pressureDevice.cu:118-194), Rayleigh damping
+ * (cuda_rayleighDampingDevice.cu:49-79), buoyancy (cuda_buoyancyDevice.cu:48-67),
+ * Coriolis with the geostrophic term (cuda_coriolisDevice.cu:50-58) and the
+ * 6th-order filter on rho*w (cuda_filtersDevice.cu:46-122).  win[f][j] holds
+ * level z-3+j; pzm/pzp are p' at z-1 and z+1; pgx/pgy the horizontal p'
+ * differences. */
+template <class C, int WD>
+__device__ __forceinline__ void dyn_tendency(const StepArgs &a, int z, float (*win)[WD], float rin, float rinp,
+                                             float pzm, float pzp, float pgx, float pgy, const float *dxy,
+                                             const float *gzlo, float *gzhi, const float *omb, float *F) {
+  constexpr int NF = C::NF;
+  const float Dz = prof(a, PR_D, z), Dp = prof(a, PR_D, z + 1);
+  const float J33z = prof(a, PR_J33, z), J33p = prof(a, PR_J33, z + 1);
+  const float invD = prof(a, PR_INVD, z);
+  const float W = (z + 1 >= a.Nz) ? 0.f : 0.5f * a.dZi * (Dp * J33p * win[3][4] * rinp + Dz * J33z * win[3][3] * rin);
+  FaceLoop<C, 0>::template zf<WD>(win, gzhi, W, omb);
+#pragma unroll
+  for (int f = 0; f < NF; ++f) F[f] = -invD * (dxy[f] + (gzhi[f] - gzlo[f]));
+  const float rho = win[0][3];
+  {
+    float fac = 1.f;
+    if constexpr (C::NM > 0) {
+      float qs = 0.f;
+#pragma unroll
+      for (int m = 0; m < C::NM; ++m) qs += win[C::IQV + m][3];
+      fac = 1.f / (1.f + qs * rin * 1e-3f);
+    }
+    F[1] -= fac * pgx;
+    F[2] -= fac * pgy;
+    F[3] -= fac * 0.5f * a.dZi * J33z * (pzp - pzm);
+  }
+  if (a.damp) F[3] -= prof(a, PR_DAMP, z) * win[3][3];
+  if constexpr (C::NM == 0) {
+    F[3] -= a.g * (rho - prof(a, PR_RHO, z));
+  } else {
+    float rt = rho;
+#pragma unroll
+    for (int m = 0; m < C::NM; ++m) rt += win[C::IQV + m][3] * 1e-3f;
+    F[3] -= (rho / rt) * a.g * (rt - prof(a, PR_RHO, z));
+  }
+  {
+    float u = win[1][3] * rin, v = win[2][3] * rin, w = win[3][3] * rin;
+    F[1] += a.fH * (v - prof(a, PR_VG, z)) - a.fV * w;
+    F[2] -= a.fH * (u - prof(a, PR_UG, z));
+    F[3] += a.fV * (u - prof(a, PR_UG, z));
+  }
+  if (a.filt != 0.f) {
+    const float *w = win[3];
+    float gm = w[3] - w[2], gp = w[4] - w[3];
+    float fm = 10.f * (w[3] - w[2]) - 5.f * (w[4] - w[1]) + (w[5] - w[0]);
+    float fp = 10.f * (w[4] - w[3]) - 5.f * (w[5] - w[2]) + (w[6] - w[1]);
+    fm = signbit(gm * fm) ? 0.f : fm;
+    fp = signbit(gp * fp) ? 0.f : fp;
+    F[3] += a.filt * (fp - fm);
+  }
+}
+
+/* RK3-WS2002 stage update of the own cell (cuda_RKschemes.cu:113-157, positivity of
+ * rho*e and moisture), or the tendency itself in emit mode. */
+template <class C>
+__device__ __forceinline__ void rk_store(const StepArgs &a, char *const *fout, unsigned oz, const float *F,
+                                         const float *base) {
+  if (a.emitF) {
+#pragma unroll
+    for (int f = 0; f < C::NF; ++f) *(float *)(fout[f] + oz) = F[f];
+  } else {
+#pragma unroll
+    for (int f = 0; f < C::NF; ++f) {
+      float v = fmaf(a.coefdt, F[f], base[f]);
+      if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
+      if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
+      *(float *)(fout[f] + oz) = v;
+    }
+  }
+}
+
 /* =============================================================================== */
 /* Halo cell h of a tile, in tile-local coordinates: the ring-1 cells (the ring
  * around the tile that the derived planes need) come first, then the rest of
@@ -343,6 +443,95 @@ __device__ __forceinline__ void sgs_lq(const StepArgs &a, int z, float thm, floa
   if (N2 > 0.f) l = fmaxf(fminf(0.76f * se * rsqrtf(N2), delta), 1e-2f);
   q = l * se;
 }
+
+/* Per-thread set-up shared by the stage kernels: tile position (XCD-aware
+ * order), own column, this thread's halo cell, per-field base pointers and the
+ * upwind weights. */
+template <class C>
+struct Tile {
+  static constexpr int NF = C::NF;
+  int tid, tx, ty, x0, y0, cx, cy, col;
+  bool active, baseIn, hasH, hasR1;
+  int hlx, hly, hcol, hslot, hd, od;
+  unsigned pb, colb, hcolb;
+  /* Global accesses use a wave-uniform per-field base pointer plus one 32-bit
+   * byte offset per thread (a field is < 4 GiB), so one offset register serves
+   * every field and the loads take the scalar-base addressing form. */
+  const char *fin[NF], *fbs[NF];
+  char *fout[NF];
+  float omb[NF];               /* (1 - b) of the hybrid schemes, 0 for the others */
+
+  __device__ __forceinline__ explicit Tile(const StepArgs &a) {
+    tid = threadIdx.x;
+    tx = tid % TX;
+    ty = tid / TX;
+    /* XCD-aware tile order: hardware round-robins work-groups over the 8 XCDs, so
+     * give each XCD a contiguous run of tiles (neighbours then share an L2). */
+    const int nbx = gridDim.x, nbt = gridDim.x * gridDim.y;
+    int b = blockIdx.x + gridDim.x * blockIdx.y;
+    if ((nbt & 7) == 0) b = (b & 7) * (nbt >> 3) + (b >> 3);
+    x0 = (b % nbx) * TX;
+    y0 = (b / nbx) * TY;
+    active = (x0 + tx < a.Nx) && (y0 + ty < a.Ny);
+    cx = wrapi(x0 + tx, a.Nx);
+    cy = wrapi(y0 + ty, a.Ny);
+    col = cy * a.P + cx;
+    baseIn = (a.base == a.in);             /* first RK stage: phi^n is the stage input */
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      fin[f] = (const char *)(a.in + (long)f * a.fs);
+      fbs[f] = (const char *)(a.base + (long)f * a.fs);
+      fout[f] = (char *)(a.out + (long)f * a.fs);
+    }
+    pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
+    /* this thread's halo cell */
+    hlx = 0; hly = 0; hcol = 0; hslot = -1;
+    hasH = tid < NHALO;
+    hasR1 = tid < NR1;
+    if (hasH) {
+      halo_cell(tid, hlx, hly);
+      hslot = (hly + HH) * RW + (hlx + HH);
+      hcol = wrapi(y0 + hly, a.Ny) * a.P + wrapi(x0 + hlx, a.Nx);
+    }
+    hd = (hly + 1) * DW + (hlx + 1);       /* ring-1 slot (valid when hasR1) */
+    od = (ty + 1) * DW + (tx + 1);         /* own slot in the ring-1 region  */
+    colb = (unsigned)col * 4u;
+    hcolb = (unsigned)hcol * 4u;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      int s = (f < 5) ? C::SH : ((C::TKE && f == C::IE) ? C::SE : (f == C::IQV ? C::SQV : C::SQL));
+      float bb = (f < 5) ? a.bh : ((C::TKE && f == C::IE) ? a.be : a.bqv);
+      omb[f] = (s == 3 || s == 2) ? bb : 0.f;
+    }
+  }
+
+  /* every field of level zz of the column at byte offset cb, with the vertical
+   * boundary rules applied for levels outside [0, Nz) (one uniform branch) */
+  __device__ __forceinline__ void lvl(const StepArgs &a, float (&v)[NF], int zz, unsigned cb) const {
+    if (zz >= 0 && zz < a.Nz) {
+      const unsigned o = (unsigned)zz * pb + cb;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) v[f] = *(const float *)(fin[f] + o);
+    } else {
+      const bool top = zz >= a.Nz;
+      const unsigned o = top ? (unsigned)(a.Nz - 1) * pb + cb : cb;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if (f == 0) v[f] = prof(a, PR_RHO, zz);
+        else if (f == 4) v[f] = prof(a, PR_RTH, zz);
+        else if (top && f == 1) v[f] = a.uTop;
+        else if (top && f == 2) v[f] = a.vTop;
+        else if (top && (f == 3 || (C::TKE && f == C::IE))) v[f] = 0.f;
+        else v[f] = *(const float *)(fin[f] + o);
+      }
+    }
+  }
+  /* moisture factor of theta_m: (R_v/R) q_v */
+  __device__ __forceinline__ float mfac(const StepArgs &a, float rho, float qv) const {
+    if constexpr (C::NM > 0) return a.Rv_Rg * qv * frcp(rho) * 1e-3f;
+    else return 0.f;
+  }
+};
 
 /* One fused RK stage.  Per level z, with two work-group barriers:
  *   P1  write the xy plane of level z (own cell from the register window,
@@ -389,79 +578,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   typedef float Plane[DH][DW];
   auto VL = [&](int lev, int q) -> Plane & { return q < 3 ? velm[lev & 3][q] : vels[(lev + 3) % 3][q - 3]; };
 
-  const int tid = threadIdx.x;
-  const int tx = tid % TX, ty = tid / TX, wave = ty;
-  /* XCD-aware tile order: hardware round-robins work-groups over the 8 XCDs, so
-   * give each XCD a contiguous run of tiles (neighbours then share an L2). */
-  const int nbx = gridDim.x, nbt = gridDim.x * gridDim.y;
-  int b = blockIdx.x + gridDim.x * blockIdx.y;
-  if ((nbt & 7) == 0) b = (b & 7) * (nbt >> 3) + (b >> 3);
-  const int x0 = (b % nbx) * TX, y0 = (b / nbx) * TY;
-  const bool active = (x0 + tx < a.Nx) && (y0 + ty < a.Ny);
-  const int cx = wrapi(x0 + tx, a.Nx), cy = wrapi(y0 + ty, a.Ny);
-  const int col = cy * a.P + cx;
-  const int Nz = a.Nz;
-  const bool baseIn = (a.base == a.in);             /* first RK stage: phi^n is the stage input */
-  /* Global accesses use a wave-uniform per-field base pointer plus one 32-bit
-   * byte offset per thread (a field is < 4 GiB), so one offset register serves
-   * every field and the loads take the scalar-base addressing form. */
-  const char *fin[NF], *fbs[NF];
-  char *fout[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    fin[f] = (const char *)(a.in + (long)f * a.fs);
-    fbs[f] = (const char *)(a.base + (long)f * a.fs);
-    fout[f] = (char *)(a.out + (long)f * a.fs);
-  }
-  const unsigned pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
+  const Tile<C> T(a);
+  const int tid = T.tid, tx = T.tx, ty = T.ty, wave = T.ty;
+  const int x0 = T.x0, y0 = T.y0, cx = T.cx, col = T.col, Nz = a.Nz;
+  const bool active = T.active, baseIn = T.baseIn, hasH = T.hasH, hasR1 = T.hasR1;
+  const int hslot = T.hslot, hd = T.hd, od = T.od;
+  const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
+  const char *const *fbs = T.fbs;
+  char *const *fout = T.fout;
+  const float *omb = T.omb;
   auto ld = [](const char *bp, unsigned off) -> float { return *(const float *)(bp + off); };
-
-  /* this thread's halo cell */
-  int hlx = 0, hly = 0, hcol = 0, hslot = -1;
-  const bool hasH = tid < NHALO;
-  const bool hasR1 = tid < NR1;
-  if (hasH) {
-    halo_cell(tid, hlx, hly);
-    hslot = (hly + HH) * RW + (hlx + HH);
-    hcol = wrapi(y0 + hly, a.Ny) * a.P + wrapi(x0 + hlx, a.Nx);
-  }
-  const int hd = (hly + 1) * DW + (hlx + 1);           /* ring-1 slot (valid when hasR1) */
-  const unsigned colb = (unsigned)col * 4u, hcolb = (unsigned)hcol * 4u;
-  const int od = (ty + 1) * DW + (tx + 1);             /* own slot in the ring-1 region  */
-
-  /* every field of level zz of the column at byte offset cb, with the vertical
-   * boundary rules applied for levels outside [0, Nz) (one uniform branch) */
-  auto lvl = [&](float (&v)[NF], int zz, unsigned cb) {
-    if (zz >= 0 && zz < Nz) {
-      const unsigned o = (unsigned)zz * pb + cb;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) v[f] = ld(fin[f], o);
-    } else {
-      const bool top = zz >= Nz;
-      const unsigned o = top ? (unsigned)(Nz - 1) * pb + cb : cb;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        if (f == 0) v[f] = prof(a, PR_RHO, zz);
-        else if (f == 4) v[f] = prof(a, PR_RTH, zz);
-        else if (top && f == 1) v[f] = a.uTop;
-        else if (top && f == 2) v[f] = a.vTop;
-        else if (top && (f == 3 || (C::TKE && f == C::IE))) v[f] = 0.f;
-        else v[f] = ld(fin[f], o);
-      }
-    }
-  };
-  auto mfac = [&](float rho, float qv) {                  /* moisture factor of theta_m */
-    if constexpr (C::NM > 0) return a.Rv_Rg * qv * frcp(rho) * 1e-3f;
-    else return 0.f;
-  };
-
-  float omb[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    int s = (f < 5) ? C::SH : ((C::TKE && f == C::IE) ? C::SE : (f == C::IQV ? C::SQV : C::SQL));
-    float bb = (f < 5) ? a.bh : ((C::TKE && f == C::IE) ? a.be : a.bqv);
-    omb[f] = (s == 3 || s == 2) ? bb : 0.f;
-  }
+  auto lvl = [&](float (&v)[NF], int zz, unsigned cb) { T.lvl(a, v, zz, cb); };
+  auto mfac = [&](float rho, float qv) { return T.mfac(a, rho, qv); };
 
   /* register window of the own column, z-3..z+3 */
   float win[NF][7];
@@ -571,24 +699,31 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     }
   }
 
-  for (int z = 0; z < Nz; ++z) {
-    /* ---- loads, in order of first use (vector-memory waits are in order):
-     *      RK base of level z and, in the plain stage, own level z+3 (both used
-     *      in this level's P3); then the halo of level z+PD and the own level
-     *      z+4 of the physics stage (used in later levels) ---- */
-    float nxt[NF], bse[NF];
-    const unsigned oz = (unsigned)z * pb + colb;
-    if (!a.emitF && !baseIn) {
+  float bse[NF], bsn[NF];              /* RK base of levels z and z+1 */
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bse[f] = ld(fbs[f], oz);
-    }
+  for (int f = 0; f < NF; ++f) bse[f] = bsn[f] = 0.f;
+  if (!a.emitF && !baseIn) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) bse[f] = ld(fbs[f], colb);
+  }
+
+  for (int z = 0; z < Nz; ++z) {
+    /* ---- loads, in order of first use (vector-memory waits are in order): own
+     *      level z+4 (window shift at the end of this level), the halo of level
+     *      z+PD, the RK base of level z+1 (next level's P3) ---- */
+    float nxt[NF];
+    const unsigned oz = (unsigned)z * pb + colb;
     if constexpr (OD == 3) {
       lvl(nxt, z + 3, colb);
 #pragma unroll
       for (int f = 0; f < NF; ++f) win[f][6] = nxt[f];
     }
-    if (hasH) lvl(hv[PD], z + PD, hcolb);
     if constexpr (OD == 4) lvl(nxt, z + 4, colb);
+    if (hasH) lvl(hv[PD], z + PD, hcolb);
+    if (z + 1 < Nz && !a.emitF && !baseIn) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) bsn[f] = ld(fbs[f], oz + pb);
+    }
 
     /* ---- P1 ---- */
 #pragma unroll
@@ -655,15 +790,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float V = hyD * (raw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + raw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
         FaceLoop<C, 0>::yf(raw, gy, ty, tx, V, omb);
       }
-      /* ring faces: wave 0 the y face row y=TY, wave 1 (lanes < TY) the x face column x=TX */
-      if (wave == 0) {
-        const int fx = tx, fy = TY;
-        float V = hyD * (raw[2][fy + HH - 1][fx + HH] * dr[fy][fx + 1] + raw[2][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
-        FaceLoop<C, 0>::yf(raw, gy, fy, fx, V, omb);
-      } else if (wave == 1 && tx < TY) {
-        const int fx = TX, fy = tx;
-        float U = hxD * (raw[1][fy + HH][fx + HH - 1] * dr[fy + 1][fx] + raw[1][fy + HH][fx + HH] * dr[fy + 1][fx + 1]);
-        FaceLoop<C, 0>::xf(raw, gx, fy, fx, U, omb);
+      /* ring faces, half of the fields per wave, on the waves without ring-1 cells
+       * (those carry the ring's rho^-1, p', velocities and Km in the other phases) */
+      constexpr int HF = NF / 2;
+      switch (wave) {
+        case 3: ring_faces_y<C, 0, HF>(raw, gy, dr, tx, hyD, omb); break;
+        case 4: ring_faces_y<C, HF, NF>(raw, gy, dr, tx, hyD, omb); break;
+        case 5: ring_faces_x<C, 0, HF>(raw, gx, dr, tx, hxD, omb); break;
+        case 6: ring_faces_x<C, HF, NF>(raw, gx, dr, tx, hxD, omb); break;
+        default: break;
       }
       if constexpr (PH) {
         const float J33z = prof(a, PR_J33, z);
@@ -732,9 +867,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             tb2[4 + m][ty][tx] = -Bv * a.dYi * (vz(4 + m)[Y][X] - vz(4 + m)[Y - 1][X]);
           }
         }
-        if (wave == 2) {
+        if (wave == 7) {
           yface(tx, TY, z == 0 ? wrapi(y0 + TY, a.Ny) * a.P + cx : 0);
-        } else if (wave == 3 && tx < TY) {
+        } else if (wave == 2 && tx < TY) {
           xface(TX, tx, z == 0 ? wrapi(y0 + tx, a.Ny) * a.P + wrapi(x0 + TX, a.Nx) : 0);
         }
         if constexpr (C::TKE) {
@@ -746,12 +881,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           auto eown = [&](int lx, int ly) { return raw[C::IE][ly + HH][lx + HH] * dr[ly + 1][lx + 1]; };
           qx[ty][tx + 1] = 0.5f * a.dXi * (eown(tx + 1, ty) - eown(tx - 1, ty)) * kmp[0][ty + 1][tx + 1];
           qy[ty + 1][tx] = 0.5f * a.dYi * (eown(tx, ty + 1) - eown(tx, ty - 1)) * kmp[0][ty + 1][tx + 1];
-          if (wave == 4) qy[0][tx] = qyf(tx, -1);
-          else if (wave == 5) qy[TY + 1][tx] = qyf(tx, TY);
-          else if (wave == 6 && tx < 2 * TY) {
-            int ly = tx % TY, side = tx / TY;
-            if (side == 0) qx[ly][0] = qxf(-1, ly);
-            else qx[ly][TX + 1] = qxf(TX, ly);
+          if (wave == 0) {
+            qy[0][tx] = qyf(tx, -1);
+            if (tx < 2 * TY) {
+              int ly = tx % TY, side = tx / TY;
+              if (side == 0) qx[ly][0] = qxf(-1, ly);
+              else qx[ly][TX + 1] = qxf(TX, ly);
+            }
+          } else if (wave == 1) {
+            qy[TY + 1][tx] = qyf(tx, TY);
           }
         }
       }
@@ -760,55 +898,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 
     /* ---- P3: own cell ---- */
     {
-      const float Dz = prof(a, PR_D, z), Dp = prof(a, PR_D, z + 1);
       const float J33z = prof(a, PR_J33, z), J33p = prof(a, PR_J33, z + 1);
-      const float invD = prof(a, PR_INVD, z);
-      float rinp = frcp(win[0][4]);
-      float W = (z + 1 >= Nz) ? 0.f : 0.5f * a.dZi * (Dp * J33p * win[3][4] * rinp + Dz * J33z * win[3][3] * rin);
-      float gzhi[NF];
-      FaceLoop<C, 0>::zf(win, gzhi, W, omb);
-      float F[NF];
+      const float rinp = frcp(win[0][4]);
+      float dxy[NF], gzhi[NF], F[NF];
 #pragma unroll
-      for (int f = 0; f < NF; ++f)
-        F[f] = -invD * ((gx[f][ty][tx + 1] - gx[f][ty][tx]) + (gy[f][ty + 1][tx] - gy[f][ty][tx]) + (gzhi[f] - gzlo[f]));
+      for (int f = 0; f < NF; ++f) dxy[f] = (gx[f][ty][tx + 1] - gx[f][ty][tx]) + (gy[f][ty + 1][tx] - gy[f][ty][tx]);
+      dyn_tendency<C, 7>(a, z, win, rin, rinp, pz[0], pz[2], pgx, pgy, dxy, gzlo, gzhi, omb, F);
       const float rho = win[0][3];
       const int X = tx + 1, Y = ty + 1;
-      {
-        float fac = 1.f;
-        if constexpr (C::NM > 0) {
-          float qs = 0.f;
-#pragma unroll
-          for (int m = 0; m < C::NM; ++m) qs += win[C::IQV + m][3];
-          fac = 1.f / (1.f + qs * rin * 1e-3f);
-        }
-        F[1] -= fac * pgx;
-        F[2] -= fac * pgy;
-        F[3] -= fac * 0.5f * a.dZi * J33z * (pz[2] - pz[0]);
-      }
-      if (a.damp) F[3] -= prof(a, PR_DAMP, z) * win[3][3];
-      if constexpr (C::NM == 0) {
-        F[3] -= a.g * (rho - prof(a, PR_RHO, z));
-      } else {
-        float rt = rho;
-#pragma unroll
-        for (int m = 0; m < C::NM; ++m) rt += win[C::IQV + m][3] * 1e-3f;
-        F[3] -= (rho / rt) * a.g * (rt - prof(a, PR_RHO, z));
-      }
-      {
-        float u = win[1][3] * rin, v = win[2][3] * rin, w = win[3][3] * rin;
-        F[1] += a.fH * (v - prof(a, PR_VG, z)) - a.fV * w;
-        F[2] -= a.fH * (u - prof(a, PR_UG, z));
-        F[3] += a.fV * (u - prof(a, PR_UG, z));
-      }
-      if (a.filt != 0.f) {
-        const float *w = win[3];
-        float gm = w[3] - w[2], gp = w[4] - w[3];
-        float fm = 10.f * (w[3] - w[2]) - 5.f * (w[4] - w[1]) + (w[5] - w[0]);
-        float fp = 10.f * (w[4] - w[3]) - 5.f * (w[5] - w[2]) + (w[6] - w[1]);
-        fm = signbit(gm * fm) ? 0.f : fm;
-        fp = signbit(gp * fp) ? 0.f : fp;
-        F[3] += a.filt * (fp - fm);
-      }
 
       if constexpr (PH) {
         auto vz = [&](int q) -> const Plane & { return VL(z, q); };
@@ -914,18 +1011,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       }
 
       if (active) {
-        if (a.emitF) {
+        float b0[NF];
 #pragma unroll
-          for (int f = 0; f < NF; ++f) *(float *)(fout[f] + oz) = F[f];
-        } else {
-#pragma unroll
-          for (int f = 0; f < NF; ++f) {
-            float v = fmaf(a.coefdt, F[f], baseIn ? win[f][3] : bse[f]);
-            if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
-            if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
-            *(float *)(fout[f] + oz) = v;
-          }
-        }
+        for (int f = 0; f < NF; ++f) b0[f] = baseIn ? win[f][3] : bse[f];
+        rk_store<C>(a, fout, oz, F, b0);
         if (a.pout) *(float *)((char *)a.pout + oz) = pz[1];
       }
 #pragma unroll
@@ -960,6 +1049,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
       for (int j = 0; j < 6; ++j) win[f][j] = win[f][j + 1];
       if constexpr (OD == 4) win[f][6] = nxt[f];
+      bse[f] = bsn[f];
     }
 #pragma unroll
     for (int l = 0; l < PD; ++l)
@@ -976,6 +1066,168 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       float q = C::NM > 0 ? win[C::IQV][4] : 0.f;
       pz[2] = pprime(a, win[4][4], mfac(win[0][4], q), z + 2);
     }
+  }
+}
+
+/* The stage without SGS physics (the first two RK stages when the physics runs once
+ * per step), software-pipelined over levels with one work-group barrier per level:
+ * iteration it writes the xy plane of level it (P1), computes the x/y face fluxes of
+ * level it-1 (P2) and the own-cell tendency and RK update of level it-2 (P3).  The
+ * plane, rho^-1, p' and face-flux buffers have two copies selected by level parity,
+ * so the three phases of one iteration never touch the same copy, and the barrier at
+ * the end of an iteration orders each copy's writes before its reads.  Every global
+ * load is issued one iteration before the iteration that first uses it; the own
+ * column and the halo cell are both loaded two levels ahead of P1, so a tile fetches
+ * its halo lines while its neighbours fetch the same lines as their own columns. */
+template <class C>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAIN_WPE, 8))) stage_plain_kernel(StepArgs a) {
+  constexpr int NF = C::NF;
+  __shared__ float raw[2][NF][RH][RW];
+  __shared__ float dvr[2][DH][DW];
+  __shared__ float dvp[2][DH][DW];
+  __shared__ float gx[2][NF][TY][TX + 1];
+  __shared__ float gy[2][NF][TY + 1][TX];
+
+  const Tile<C> T(a);
+  const int tx = T.tx, ty = T.ty, wave = T.ty, Nz = a.Nz;
+  const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
+
+  /* register window of the own column: win[f][j] holds level it-5+j, so P3's level
+   * it-2 sits at j = 3, P1's level it at j = 5, and j = 7 is the level in flight */
+  float win[NF][8];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    float t[NF];
+    if (j >= 2) {
+      T.lvl(a, t, j - 5, colb);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels -5, -4: never read */
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) win[f][j] = t[f];
+  }
+  float hv[3][NF];                                   /* halo cell, levels it .. it+2 */
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    if (T.hasH && l < 2) {
+      T.lvl(a, hv[l], l, hcolb);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
+    }
+  }
+  float bse[NF], bsn[NF];                            /* RK base of levels it-2 and it-1 */
+#pragma unroll
+  for (int f = 0; f < NF; ++f) bse[f] = bsn[f] = 0.f;
+  float p4[4] = {0.f, 0.f, 0.f, 0.f};               /* own p' of levels it-3 .. it */
+  float ri[3] = {0.f, 0.f, 0.f};                     /* own rho^-1 of levels it-2 .. it */
+  float gzlo[NF];
+#pragma unroll
+  for (int f = 0; f < NF; ++f) gzlo[f] = 0.f;
+  float pgc[2] = {0.f, 0.f}, pgn[2] = {0.f, 0.f};    /* horizontal p' differences, levels it-2 and it-1 */
+
+  for (int it = 0; it < Nz + 2; ++it) {
+    const int z1 = it, z2 = it - 1, z3 = it - 2;
+    const int b1 = it & 1, b2 = b1 ^ 1;              /* copies of levels it (and it-2) and it-1 */
+    const unsigned o3 = (unsigned)z3 * pb + colb;
+
+    /* ---- loads for the next iteration: RK base of level it-1, own level it+2,
+     *      halo level it+2 ---- */
+    if (z2 >= 0 && z2 < Nz && !a.emitF && !T.baseIn) {
+      const unsigned o2 = (unsigned)z2 * pb + colb;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) bsn[f] = *(const float *)(T.fbs[f] + o2);
+    }
+    {
+      float t[NF];
+      T.lvl(a, t, it + 2, colb);
+#pragma unroll
+      for (int f = 0; f < NF; ++f) win[f][7] = t[f];
+    }
+    if (T.hasH && it + 2 < Nz) T.lvl(a, hv[2], it + 2, hcolb);
+
+    /* ---- P1: plane, rho^-1 and p' of level it ---- */
+    ri[2] = frcp(win[0][5]);
+    if (z1 < Nz) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) raw[b1][f][ty + HH][tx + HH] = win[f][5];
+      if (T.hasH) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) (&raw[b1][f][0][0])[T.hslot] = hv[0][f];
+      }
+      const float q = C::NM > 0 ? win[C::IQV][5] : 0.f;
+      p4[3] = pprime(a, win[4][5], T.mfac(a, win[0][5], q), z1);
+      if (it == 0) p4[2] = p4[3];                    /* p'(-1) = p'(0) */
+      (&dvr[b1][0][0])[T.od] = ri[2];
+      (&dvp[b1][0][0])[T.od] = p4[3];
+      if (T.hasR1) {
+        const float qh = C::NM > 0 ? hv[0][C::IQV] : 0.f;
+        (&dvr[b1][0][0])[T.hd] = frcp(hv[0][0]);
+        (&dvp[b1][0][0])[T.hd] = pprime(a, hv[0][4], T.mfac(a, hv[0][0], qh), z1);
+      }
+    } else {
+      p4[3] = p4[2];                                 /* p'(Nz) = p'(Nz-1) */
+    }
+
+    /* ---- P2: x and y face fluxes of level it-1 ---- */
+    if (z2 >= 0 && z2 < Nz) {
+      float (*rw)[RH][RW] = raw[b2];
+      const float (*dr)[DW] = dvr[b2];
+      pgn[0] = 0.5f * a.dXi * (dvp[b2][ty + 1][tx + 2] - dvp[b2][ty + 1][tx]);
+      pgn[1] = 0.5f * a.dYi * (dvp[b2][ty + 2][tx + 1] - dvp[b2][ty][tx + 1]);
+      const float Dz = prof(a, PR_D, z2);
+      const float hxD = 0.5f * a.dXi * Dz, hyD = 0.5f * a.dYi * Dz;
+      {
+        float U = hxD * (rw[1][ty + HH][tx + HH - 1] * dr[ty + 1][tx] + rw[1][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
+        FaceLoop<C, 0>::xf(rw, gx[b2], ty, tx, U, T.omb);
+        float V = hyD * (rw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + rw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
+        FaceLoop<C, 0>::yf(rw, gy[b2], ty, tx, V, T.omb);
+      }
+      /* ring faces, a third of the fields per wave: waves 0-2 the y row, 3-5 the x column */
+      constexpr int A = NF / 3, B = 2 * NF / 3;
+      switch (wave) {
+        case 0: ring_faces_y<C, 0, A>(rw, gy[b2], dr, tx, hyD, T.omb); break;
+        case 1: ring_faces_y<C, A, B>(rw, gy[b2], dr, tx, hyD, T.omb); break;
+        case 2: ring_faces_y<C, B, NF>(rw, gy[b2], dr, tx, hyD, T.omb); break;
+        case 3: ring_faces_x<C, 0, A>(rw, gx[b2], dr, tx, hxD, T.omb); break;
+        case 4: ring_faces_x<C, A, B>(rw, gx[b2], dr, tx, hxD, T.omb); break;
+        case 5: ring_faces_x<C, B, NF>(rw, gx[b2], dr, tx, hxD, T.omb); break;
+        default: break;
+      }
+    }
+
+    /* ---- P3: own cell of level it-2 ---- */
+    if (z3 >= 0) {
+      float dxy[NF], gzhi[NF], F[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        dxy[f] = (gx[b1][f][ty][tx + 1] - gx[b1][f][ty][tx]) + (gy[b1][f][ty + 1][tx] - gy[b1][f][ty][tx]);
+      dyn_tendency<C, 8>(a, z3, win, ri[0], ri[1], p4[0], p4[2], pgc[0], pgc[1], dxy, gzlo, gzhi, T.omb, F);
+      if (T.active) {
+        float b0[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3] : bse[f];
+        rk_store<C>(a, T.fout, o3, F, b0);
+        if (a.pout) *(float *)((char *)a.pout + o3) = p4[1];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) gzlo[f] = gzhi[f];
+    }
+    __syncthreads();
+
+    /* ---- advance the pipelines ---- */
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+      for (int j = 0; j < 7; ++j) win[f][j] = win[f][j + 1];
+      hv[0][f] = hv[1][f];
+      hv[1][f] = hv[2][f];
+      bse[f] = bsn[f];
+    }
+    p4[0] = p4[1]; p4[1] = p4[2]; p4[2] = p4[3];
+    ri[0] = ri[1]; ri[1] = ri[2];
+    pgc[0] = pgn[0]; pgc[1] = pgn[1];
   }
 }
 

@@ -76,6 +76,12 @@ int fe_timing_read(double *ms, int *count, int ncls);
 double fe_copy_probe(long nfloats, int reps);
 /* Name of the kernel variant selected for the stage-1/2 and stage-3 tendencies. */
 const char *fe_kernel_info(void);
+/* Horizontal-mean profile statistics of the current device state, computed on the
+ * device in FP64: out[Nz][nstat] with nstat = 14 columns <u>, <v>, <w>, <theta>,
+ * <u'u'>, <u'w'>, <v'v'>, <v'w'>, <w'w'>, resolved TKE, <theta'w'>, <e>, <q_v>, <q_l>.
+ * No counterpart in the reference's C code; it restates the reference's
+ * post-processing (tutorials/notebooks/fetutorialfunctions.py:55-83,344-431). */
+int fe_profiles(double *out, int nstat);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,164 @@
+"""ctypes wrapper of the CPU oracle (liboracle_f32.so / liboracle_f64.so in this directory).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and the CPU baseline; the product
+library never loads it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class FeCfg(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("Nh", C.c_int),
+        ("d_xi", C.c_double), ("d_eta", C.c_double), ("d_zeta", C.c_double),
+        ("hydroBCs", C.c_int),
+        ("advectionSelector", C.c_int), ("b_hyb", C.c_double),
+        ("TKEAdvSelector", C.c_int), ("TKEAdvSelector_b_hyb", C.c_double),
+        ("pgfSelector", C.c_int), ("buoyancySelector", C.c_int), ("coriolisSelector", C.c_int),
+        ("coriolisLatitude", C.c_double),
+        ("turbulenceSelector", C.c_int), ("TKESelector", C.c_int),
+        ("c_s", C.c_double), ("c_k", C.c_double),
+        ("surflayerSelector", C.c_int),
+        ("surflayer_z0", C.c_double), ("surflayer_z0t", C.c_double), ("surflayer_wth", C.c_double),
+        ("surflayer_tr", C.c_double), ("surflayer_wq", C.c_double), ("surflayer_qr", C.c_double),
+        ("surflayer_idealsine", C.c_int),
+        ("surflayer_ideal_ts", C.c_double), ("surflayer_ideal_te", C.c_double), ("surflayer_ideal_amp", C.c_double),
+        ("surflayer_ideal_qts", C.c_double), ("surflayer_ideal_qte", C.c_double), ("surflayer_ideal_qamp", C.c_double),
+        ("surflayer_stab", C.c_int),
+        ("moistureSelector", C.c_int), ("moistureNvars", C.c_int), ("moistureAdvSelectorQv", C.c_int),
+        ("moistureAdvSelectorQv_b", C.c_double),
+        ("moistureAdvSelectorQi", C.c_int), ("moistureSGSturb", C.c_int), ("moistureCond", C.c_int),
+        ("moistureCondTscale", C.c_double),
+        ("moistureCondBasePres", C.c_int),
+        ("moistureMPcallTscale", C.c_double),
+        ("filterSelector", C.c_int), ("filter_6th_coeff", C.c_double),
+        ("dampingLayerSelector", C.c_int), ("dampingLayerDepth", C.c_double),
+        ("temp_grnd", C.c_double), ("pres_grnd", C.c_double),
+        ("U_g", C.c_double), ("V_g", C.c_double), ("z_Ug", C.c_double), ("z_Vg", C.c_double),
+        ("Ug_grad", C.c_double), ("Vg_grad", C.c_double),
+        ("physics_oneRKonly", C.c_int),
+        ("dt", C.c_double),
+    ]
+
+
+OR_IDS = ["HYDRO", "HYDRO_F", "TKE", "TKE_F", "MOIST", "MOIST_F", "BS", "ZPOS", "J31", "J32", "J33", "DJAC",
+          "INVDJAC", "PRES", "RHOINV", "FACEVEL", "TAU", "KM", "LS", "DEDXI", "MOISTTAU", "FCOND", "TIME0",
+          "CD", "CH", "CQ", "FRICVEL", "HTFLUX", "TSKIN", "INVOBLEN", "Z0M", "Z0T", "QFLUX", "QSKIN"]
+OR = {n: i for i, n in enumerate(OR_IDS)}
+
+_DEFAULTS = {"physics_oneRKonly": 1, "pgfSelector": 1, "buoyancySelector": 1, "surflayer_stab": 0,
+             "moistureNvars": 0, "moistureAdvSelectorQv": 0, "moistureAdvSelectorQv_b": 0.0,
+             "moistureAdvSelectorQi": 0, "moistureSGSturb": 0, "moistureCond": 1, "moistureCondTscale": 1.0,
+             "moistureCondBasePres": 0, "moistureMPcallTscale": 1.0, "surflayer_wq": 0.0, "surflayer_qr": 0.0,
+             "surflayer_ideal_ts": 0.0, "surflayer_ideal_te": 0.0, "surflayer_ideal_amp": 0.1,
+             "surflayer_ideal_qts": 0.0, "surflayer_ideal_qte": 0.0, "surflayer_ideal_qamp": 0.1}
+
+
+def make_cfg(params):
+    """fe_cfg from a parameter dict; float parameters are rounded to float32 as sscanf('%f') does."""
+    cfg = FeCfg()
+    for name, ctype in FeCfg._fields_:
+        v = params.get(name, _DEFAULTS.get(name, 0))
+        if ctype is C.c_double:
+            setattr(cfg, name, float(np.float32(v)))
+        else:
+            setattr(cfg, name, int(v))
+    return cfg
+
+
+class Oracle:
+    """CPU oracle handle (REAL=float for bits=32, REAL=double for bits=64)."""
+
+    _libs = {}
+
+    def __init__(self, params, bits=32, threads=1):
+        if bits not in Oracle._libs:
+            p = os.path.join(HERE, f"liboracle_f{bits}.so")
+            L = C.CDLL(p)
+            L.oracle_create.restype = C.c_void_p
+            L.oracle_create.argtypes = [C.POINTER(FeCfg)]
+            L.oracle_destroy.argtypes = [C.c_void_p]
+            L.oracle_array.restype = C.c_void_p
+            L.oracle_array.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
+            L.oracle_build_frhs.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_int]
+            L.oracle_rk_stage.argtypes = [C.c_void_p, C.c_int]
+            L.oracle_steps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_int]
+            L.oracle_apply_bcs.argtypes = [C.c_void_p]
+            L.oracle_set_threads.argtypes = [C.c_int]
+            Oracle._libs[bits] = L
+        self.L = Oracle._libs[bits]
+        self.L.oracle_set_threads(int(threads))
+        self.bits = bits
+        self.dtype = np.float32 if bits == 32 else np.float64
+        self.params = dict(params)
+        self.cfg = make_cfg(params)
+        self.h = self.L.oracle_create(C.byref(self.cfg))
+        if not self.h:
+            raise MemoryError("oracle_create failed")
+        nh = int(params["Nh"])
+        self.shape3 = (int(params["Nx"]) + 2 * nh, int(params["Ny"]) + 2 * nh, int(params["Nz"]) + 2 * nh)
+        self.nh = nh
+        self.simTime = 0.0
+        self.it = 0
+        self.itR = 0
+
+    def arr(self, name):
+        n = C.c_long()
+        p = self.L.oracle_array(self.h, OR[name], C.byref(n))
+        ct = C.c_float if self.bits == 32 else C.c_double
+        flat = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n.value,))
+        n3 = self.shape3[0] * self.shape3[1] * self.shape3[2]
+        if n.value >= n3 and n.value % n3 == 0:
+            return flat.reshape((n.value // n3,) + self.shape3)
+        return flat.reshape(self.shape3[:2])
+
+    def load_from_model(self, m):
+        """Copy the model's host state, base state, grid metrics and surface fields."""
+        self.arr("HYDRO")[...] = m.array("hydroFlds")
+        if m.array("sgstkeScalars") is not None:
+            self.arr("TKE")[: m.array("sgstkeScalars").shape[0]] = m.array("sgstkeScalars")
+        if m.array("moistScalars") is not None:
+            self.arr("MOIST")[: m.array("moistScalars").shape[0]] = m.array("moistScalars")
+        self.arr("BS")[...] = m.array("hydroBaseStateFlds")
+        for a, b in [("ZPOS", "zPos"), ("J31", "J31"), ("J32", "J32"), ("J33", "J33"), ("DJAC", "D_Jac"), ("INVDJAC", "invD_Jac")]:
+            self.arr(a)[0] = m.array(b)
+        for a, b in [("CD", "cdFld"), ("CH", "chFld"), ("CQ", "cqFld"), ("FRICVEL", "fricVel"), ("HTFLUX", "htFlux"),
+                     ("TSKIN", "tskin"), ("INVOBLEN", "invOblen"), ("Z0M", "z0m"), ("Z0T", "z0t"), ("QFLUX", "qFlux"),
+                     ("QSKIN", "qskin")]:
+            x = m.array(b)
+            if x is not None:
+                self.arr(a)[...] = x
+        self.simTime = m.getf("simTime")
+        self.it = m.geti("simTime_it")
+        self.itR = m.geti("simTime_itRestart")
+
+    def build_frhs(self, stage):
+        self.L.oracle_build_frhs(self.h, C.c_float(self.simTime), self.it, self.itR, int(stage))
+
+    def rk_stage(self, stage):
+        self.L.oracle_rk_stage(self.h, int(stage))
+
+    def steps(self, n):
+        st = C.c_float(self.simTime)
+        it = C.c_int(self.it)
+        self.L.oracle_steps(self.h, int(n), C.byref(st), C.byref(it), self.itR)
+        self.simTime, self.it = st.value, it.value
+
+    def apply_bcs(self):
+        self.L.oracle_apply_bcs(self.h)
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

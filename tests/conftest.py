@@ -6,6 +6,7 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(REPO, "fasteddy-model_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
 
 
 def pytest_configure(config):

@@ -23,7 +23,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "..", "..", "fasteddy-model_amd"))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
 import fe_py  # noqa: E402
+import oracle_py  # noqa: E402
 
 CASES = {
     "dry_12x10x12": (dict(fe_py.CFG0, Nx=12, Ny=10, Nz=12), 7, False),
@@ -50,7 +52,7 @@ def run_case(params, seed, moist):
             if a is not None:
                 out[host] = a.copy()
         for prec in (64, 32):
-            o = fe_py.Oracle(params, prec)
+            o = oracle_py.Oracle(params, prec)
             o.load_from_model(m)
             for stage in (0, 2):
                 o.build_frhs(stage)

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import fe_py
+import oracle_py
 
 pytestmark = pytest.mark.gpu
 NH = 3
@@ -48,7 +49,7 @@ def make(params, tmp_path, seed):
     m = fe_py.Model(params, str(tmp_path))
     fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=seed)
     m.upload()
-    o32, o64 = fe_py.Oracle(params, 32), fe_py.Oracle(params, 64)
+    o32, o64 = oracle_py.Oracle(params, 32), oracle_py.Oracle(params, 64)
     o32.load_from_model(m)
     o64.load_from_model(m)
     return m, o32, o64
@@ -112,7 +113,7 @@ def test_rk3_steps_match_oracle(tmp_path):
     m = fe_py.Model(p, str(tmp_path))
     try:
         fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=21)
-        o64 = fe_py.Oracle(p, 64)
+        o64 = oracle_py.Oracle(p, 64)
         o64.load_from_model(m)
         h0 = m.interior(m.array("hydroFlds")).astype(np.float64).copy()
         e0 = m.interior(m.array("sgstkeScalars")).astype(np.float64).copy()
@@ -143,7 +144,7 @@ def make_moist(params, tmp_path, seed):
     fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=seed)
     fe_py.moist_state(m, seed=seed + 1)
     m.upload()
-    o32, o64 = fe_py.Oracle(params, 32), fe_py.Oracle(params, 64)
+    o32, o64 = oracle_py.Oracle(params, 32), oracle_py.Oracle(params, 64)
     o32.load_from_model(m)
     o64.load_from_model(m)
     return m, o32, o64
@@ -176,7 +177,7 @@ def test_rk3_steps_moist(tmp_path):
     try:
         fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=3)
         fe_py.moist_state(m, seed=4)
-        o64 = fe_py.Oracle(p, 64)
+        o64 = oracle_py.Oracle(p, 64)
         o64.load_from_model(m)
         h0 = m.interior(m.array("hydroFlds")).astype(np.float64).copy()
         q0 = m.interior(m.array("moistScalars")).astype(np.float64).copy()
@@ -220,5 +221,79 @@ def test_stage_tendencies_against_golden(name, tmp_path):
                 e_gpu, e_ref = rel_err(g, r64), rel_err(r32, r64)
                 print(f"golden {name} stage {stage} {fhost}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
                 assert np.all(e_gpu <= np.maximum(1e-5, 2 * e_ref)), (fhost, e_gpu, e_ref)
+    finally:
+        m.close()
+
+
+def test_steps_on_full_planes(tmp_path):
+    """RK3 steps on 512x512 planes (the production plane size, few levels): every load and
+    store stays inside the state buffers (a read one plane past the last level faults at
+    this plane size), the state stays finite and total mass is conserved."""
+    p = dict(fe_py.CFG1, Nx=512, Ny=512, Nz=6, NtBatch=2)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=2)
+        m.upload()
+        rho0 = m.interior(m.array("hydroFlds"))[0].astype(np.float64).sum()
+        m.device_steps(3)
+        m.L.fe_download_state()
+        h = m.interior(m.array("hydroFlds")).astype(np.float64)
+        assert np.all(np.isfinite(h))
+        assert abs(h[0].sum() - rho0) <= 1e-6 * rho0
+    finally:
+        m.close()
+
+
+def test_device_profiles_match_restatement(tmp_path):
+    """The FP64 device reduction (fe_profiles) against the numpy restatement of the reference's
+    post-processing on the same state, after a few steps of turbulent flow."""
+    import fe_profiles
+    p = dict(fe_py.CFG0, Nx=64, Ny=40, Nz=24)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=12)
+        m.upload()
+        m.device_steps(3)
+        dev = m.profiles()
+        m.L.fe_download_state()
+        ref = fe_profiles.profiles(m.array("hydroFlds"), m.array("sgstkeScalars"))
+        scale = np.maximum(np.abs(ref).max(axis=0), 1e-30)
+        err = (np.abs(dev - ref) / scale).max(axis=0)
+        print(f"device profiles vs numpy: {err}")
+        assert np.all(err < 1e-10), err
+    finally:
+        m.close()
+
+
+def test_long_run_profiles_dry_cbl(tmp_path):
+    """The dry CBL at 64x64x48, dx = 20 m, dt = 0.04 s, 200 RK3 steps from the seeded initial
+    state, on the GPU and on the CPU oracle (FP64 and the FP32 restatement of the reference's
+    arithmetic).  Horizontal-mean profiles (level means of u, v, w, theta; covariances; resolved
+    and SGS TKE; heat flux) must agree with FP64 at least as closely as the FP32 reference
+    does, to a factor 2, or within 1e-6 of each column's range."""
+    import fe_profiles
+    p = dict(fe_py.CFG0, NtBatch=200)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        o64 = oracle_py.Oracle(p, 64, threads=16)
+        o32 = oracle_py.Oracle(p, 32, threads=16)
+        o64.load_from_model(m)
+        o32.load_from_model(m)
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        m.commence()
+        o64.steps(200)
+        o32.steps(200)
+        assert m.geti("simTime_it") == 200
+        pg = fe_profiles.profiles(m.array("hydroFlds"), m.array("sgstkeScalars"))
+        p64 = fe_profiles.profiles(o64.arr("HYDRO"), o64.arr("TKE")[:1])
+        p32 = fe_profiles.profiles(o32.arr("HYDRO"), o32.arr("TKE")[:1])
+        np.testing.assert_allclose(m.profiles(), pg, rtol=1e-9, atol=1e-12)
+        rng = np.maximum(np.abs(p64).max(axis=0), 1e-30)
+        e_gpu = (np.abs(pg - p64)).max(axis=0) / rng
+        e_ref = (np.abs(p32 - p64)).max(axis=0) / rng
+        print(f"200-step profiles: gpu-vs-fp64 {e_gpu}\n                   ref32-vs-fp64 {e_ref}")
+        assert np.all(e_gpu <= np.maximum(2 * e_ref, 1e-6)), (e_gpu, e_ref)
+        o64.close()
+        o32.close()
     finally:
         m.close()

@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 import fe_py
+import oracle_py
 
 NH = 3
 
@@ -40,7 +41,7 @@ def I(a):
 def test_rk3_ws2002_amplification():
     """y' = lam*y: one RK3-WS2002 step multiplies by 1 + z + z^2/2 + z^3/6 (cuda_RKschemes.cu:113-136)."""
     p = small(dt=0.1)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     lam = -2.3
     y0 = np.random.default_rng(0).uniform(0.5, 1.5, size=o.arr("HYDRO").shape)
     h = o.arr("HYDRO")
@@ -66,7 +67,7 @@ def test_hyb56_polynomial_exactness(bhyb):
     poly = sum(ci * (x / 100.0) ** n for n, ci in enumerate(c)) + 300.0
     dpoly = sum(n * ci * (x / 100.0) ** (n - 1) / 100.0 for n, ci in enumerate(c) if n > 0)
     U = 3.0
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     h = o.arr("HYDRO")
     h[0] = 1.0
     h[1] = U
@@ -93,7 +94,7 @@ def _sine_error(n, scheme, bhyb):
               coriolisSelector=0, turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
     x = (np.arange(n + 2 * NH) - NH + 0.5) * dx
     U = 2.0
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     h = o.arr("HYDRO")
     h[0] = 1.0
     h[1] = U
@@ -127,7 +128,7 @@ def _monotone_error(n, scheme, U):
     p = small(Nx=n, Ny=4, Nz=4, d_xi=dx, advectionSelector=scheme, b_hyb=0.0, pgfSelector=0, buoyancySelector=0,
               coriolisSelector=0, turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
     x = (np.arange(n + 2 * NH) - NH + 0.5) * dx
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     h = o.arr("HYDRO")
     h[0] = 1.0
     h[1] = U
@@ -163,7 +164,7 @@ def test_constant_fields_have_no_advective_tendency():
     for scheme in (0, 1, 2, 3, 4, 5, 6):
         p = small(advectionSelector=scheme, pgfSelector=0, buoyancySelector=0, coriolisSelector=0,
                   turbulenceSelector=0, TKESelector=0, surflayerSelector=0, dampingLayerSelector=0)
-        o = fe_py.Oracle(p, 64)
+        o = oracle_py.Oracle(p, 64)
         h = o.arr("HYDRO")
         rng = np.random.default_rng(scheme)
         h[0] = 1.1
@@ -188,7 +189,7 @@ def test_mass_conservation(tmp_path):
     p = small()
     m = host_model(p, tmp_path)
     fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=5)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     o.load_from_model(m)
     m.close()
     for st in (0, 2):
@@ -202,7 +203,7 @@ def test_geostrophic_rest_has_zero_tendency(tmp_path):
     """rho=rho_BS, rho*theta=(rho*theta)_BS, u=U_g, w=0, e=0: stage-1 F is identically zero."""
     p = small(surflayerSelector=0)
     m = host_model(p, tmp_path)
-    o = fe_py.Oracle(p, 32)
+    o = oracle_py.Oracle(p, 32)
     o.load_from_model(m)
     m.close()
     o.build_frhs(0)
@@ -216,7 +217,7 @@ def test_rayleigh_profile(tmp_path):
     p = small(surflayerSelector=0, coriolisSelector=0, buoyancySelector=0, pgfSelector=0, turbulenceSelector=0,
               TKESelector=0, dampingLayerDepth=150.0, advectionSelector=3)
     m = host_model(p, tmp_path)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     o.load_from_model(m)
     m.close()
     h = o.arr("HYDRO")
@@ -243,7 +244,7 @@ def test_rayleigh_profile(tmp_path):
 def _sfc_oracle(tmp_path, wth, stab, U=6.0):
     p = small(surflayerSelector=1, surflayer_wth=wth, surflayer_stab=stab, U_g=U, coriolisSelector=0)
     m = host_model(p, tmp_path)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     o.load_from_model(m)
     m.close()
     o.build_frhs(2)
@@ -292,7 +293,7 @@ def test_sgs_length_scale_branches_and_shear_stress():
     """e = 0 in a stable column gives l = 0.01 m; a neutral column gives l = Delta; pure shear
     u = alpha z with uniform e gives tau_13 = -2 rho Km alpha/2 with Km = c_k Delta sqrt(e)."""
     p = small(Nx=8, Ny=8, Nz=12, surflayerSelector=0, coriolisSelector=0, d_xi=10.0, d_eta=10.0, d_zeta=10.0)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     for a in ("J33", "DJAC", "INVDJAC"):
         o.arr(a)[...] = 1.0
     nz = p["Nz"] + 2 * NH
@@ -331,7 +332,7 @@ def test_sgs_length_scale_branches_and_shear_stress():
 def test_brunt_vaisala_of_linear_theta():
     """theta = theta0 + G z: N^2 = g G / theta (length scale 0.76 sqrt(e)/N when below Delta)."""
     p = small(Nx=8, Ny=8, Nz=12, surflayerSelector=0, coriolisSelector=0, d_xi=50.0, d_eta=50.0, d_zeta=50.0)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     for a in ("J33", "DJAC", "INVDJAC"):
         o.arr(a)[...] = 1.0
     nz = p["Nz"] + 2 * NH
@@ -366,7 +367,7 @@ def test_flatau_saturation_vapour_pressure(tmp_path):
     assert es(30.0) == pytest.approx(42.46, rel=2e-3)
     p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=10, thetaPerturbationSwitch=0, surflayerSelector=0)
     m = host_model(p, tmp_path)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     o.load_from_model(m)
     m.close()
     mq = o.arr("MOIST")
@@ -381,7 +382,7 @@ def test_fp32_oracle_tracks_fp64(tmp_path):
     p = small()
     m = host_model(p, tmp_path)
     fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=11)
-    o32, o64 = fe_py.Oracle(p, 32), fe_py.Oracle(p, 64)
+    o32, o64 = oracle_py.Oracle(p, 32), oracle_py.Oracle(p, 64)
     o32.load_from_model(m)
     o64.load_from_model(m)
     m.close()
@@ -397,7 +398,7 @@ def test_fp32_oracle_tracks_fp64(tmp_path):
 
 def _uniform_oracle(p, rho=1.0, rth=300.0):
     """FP64 oracle on a flat unit-metric grid with a uniform base state."""
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     for a in ("J33", "DJAC", "INVDJAC"):
         o.arr(a)[...] = 1.0
     nz = p["Nz"] + 2 * NH
@@ -502,7 +503,7 @@ def test_saturation_adjustment_threshold_and_budgets(tmp_path):
     p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=10, thetaPerturbationSwitch=0, surflayerSelector=0, coriolisSelector=0,
              dampingLayerSelector=0)
     m = host_model(p, tmp_path)
-    o = fe_py.Oracle(p, 64)
+    o = oracle_py.Oracle(p, 64)
     o.load_from_model(m)
     m.close()
     f32 = lambda v: float(np.float32(v))

@@ -186,9 +186,7 @@ struct FaceLoop {
 };
 
 /* Advective fluxes of fields [F0, F1) through the tile's ring faces: the y face
- * row y = TY (one face per lane) and the x face column x = TX (lanes < TY).  The
- * ring is split by field over several waves so that no wave carries a whole
- * extra face pass. */
+ * row y = TY (one face per lane) and the x face column x = TX (lanes < TY). */
 template <class C, int F0, int F1>
 __device__ __forceinline__ void ring_faces_y(float (*raw)[RH][RW], float (*gy)[TY + 1][TX], const float (*dr)[DW], int tx,
                                              float hyD, const float *omb) {
@@ -1184,17 +1182,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
         float V = hyD * (rw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + rw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
         FaceLoop<C, 0>::yf(rw, gy[b2], ty, tx, V, T.omb);
       }
-      /* ring faces, a third of the fields per wave: waves 0-2 the y row, 3-5 the x column */
-      constexpr int A = NF / 3, B = 2 * NF / 3;
-      switch (wave) {
-        case 0: ring_faces_y<C, 0, A>(rw, gy[b2], dr, tx, hyD, T.omb); break;
-        case 1: ring_faces_y<C, A, B>(rw, gy[b2], dr, tx, hyD, T.omb); break;
-        case 2: ring_faces_y<C, B, NF>(rw, gy[b2], dr, tx, hyD, T.omb); break;
-        case 3: ring_faces_x<C, 0, A>(rw, gx[b2], dr, tx, hxD, T.omb); break;
-        case 4: ring_faces_x<C, A, B>(rw, gx[b2], dr, tx, hxD, T.omb); break;
-        case 5: ring_faces_x<C, B, NF>(rw, gx[b2], dr, tx, hxD, T.omb); break;
-        default: break;
-      }
+      /* ring faces: wave 0 the y row y = TY, wave 1 the x column x = TX (splitting them
+       * over more waves costs more issue slots than it saves in barrier wait) */
+      if (wave == 0) ring_faces_y<C, 0, NF>(rw, gy[b2], dr, tx, hyD, T.omb);
+      else if (wave == 1) ring_faces_x<C, 0, NF>(rw, gx[b2], dr, tx, hxD, T.omb);
     }
 
     /* ---- P3: own cell of level it-2 ---- */

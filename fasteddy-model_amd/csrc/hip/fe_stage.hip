@@ -23,6 +23,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <type_traits>
 #include "fe_kernels.h"
 
 namespace fe {
@@ -1090,24 +1091,32 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
   const int tx = T.tx, ty = T.ty, wave = T.ty, Nz = a.Nz;
   const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
 
-  /* register window of the own column: win[f][j] holds level it-5+j, so P3's level
-   * it-2 sits at j = 3, P1's level it at j = 5, and j = 7 is the level in flight */
-  float win[NF][8];
+  /* Loop-carried values live in small register arrays that the body indexes with a
+   * compile-time slot offset S: the loop runs two levels per trip (S = 0, 1) and
+   * rotates the arrays by two slots at the end of the trip, instead of shifting
+   * every array by one slot per level.
+   *   win[f][j+S]: own column, level it-5+j (P3's level it-2 at j = 3, P1's level it
+   *                at j = 5, j = 7 in flight);
+   *   hv[l+S]: halo cell, level it+l;  bs[S]: RK base of level it-2 (bs[S+1] in flight);
+   *   pp[S..S+3]: own p' of levels it-3..it;  ri[S..S+2]: own rho^-1 of levels it-2..it;
+   *   pg[S]: horizontal p' differences of level it-2 (pg[S+1] written for it-1);
+   *   gz[S]: z flux through the lower face of level it-2 (gz[S+1] its upper face). */
+  float win[NF][9];
 #pragma unroll
-  for (int j = 0; j < 7; ++j) {
+  for (int j = 0; j < 9; ++j) {
     float t[NF];
-    if (j >= 2) {
+    if (j >= 2 && j < 7) {
       T.lvl(a, t, j - 5, colb);
     } else {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels -5, -4: never read */
+      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels -5, -4: never read; 7, 8: loaded */
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) win[f][j] = t[f];
   }
-  float hv[3][NF];                                   /* halo cell, levels it .. it+2 */
+  float hv[4][NF];
 #pragma unroll
-  for (int l = 0; l < 3; ++l) {
+  for (int l = 0; l < 4; ++l) {
     if (T.hasH && l < 2) {
       T.lvl(a, hv[l], l, hcolb);
     } else {
@@ -1115,65 +1124,64 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
       for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
     }
   }
-  float bse[NF], bsn[NF];                            /* RK base of levels it-2 and it-1 */
+  float bs[3][NF], gz[3][NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) bse[f] = bsn[f] = 0.f;
-  float p4[4] = {0.f, 0.f, 0.f, 0.f};               /* own p' of levels it-3 .. it */
-  float ri[3] = {0.f, 0.f, 0.f};                     /* own rho^-1 of levels it-2 .. it */
-  float gzlo[NF];
+  for (int l = 0; l < 3; ++l)
 #pragma unroll
-  for (int f = 0; f < NF; ++f) gzlo[f] = 0.f;
-  float pgc[2] = {0.f, 0.f}, pgn[2] = {0.f, 0.f};    /* horizontal p' differences, levels it-2 and it-1 */
+    for (int f = 0; f < NF; ++f) bs[l][f] = gz[l][f] = 0.f;
+  float pp[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float ri[4] = {0.f, 0.f, 0.f, 0.f};
+  float pg[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
 
-  for (int it = 0; it < Nz + 2; ++it) {
+  auto step = [&](auto slot, int it) {
+    constexpr int S = decltype(slot)::value;
     const int z1 = it, z2 = it - 1, z3 = it - 2;
     const int b1 = it & 1, b2 = b1 ^ 1;              /* copies of levels it (and it-2) and it-1 */
     const unsigned o3 = (unsigned)z3 * pb + colb;
 
-    /* ---- loads for the next iteration: RK base of level it-1, own level it+2,
-     *      halo level it+2 ---- */
+    /* ---- loads for the next level: RK base of level it-1, own level it+2, halo level it+2 ---- */
     if (z2 >= 0 && z2 < Nz && !a.emitF && !T.baseIn) {
       const unsigned o2 = (unsigned)z2 * pb + colb;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bsn[f] = *(const float *)(T.fbs[f] + o2);
+      for (int f = 0; f < NF; ++f) bs[S + 1][f] = *(const float *)(T.fbs[f] + o2);
     }
     {
       float t[NF];
       T.lvl(a, t, it + 2, colb);
 #pragma unroll
-      for (int f = 0; f < NF; ++f) win[f][7] = t[f];
+      for (int f = 0; f < NF; ++f) win[f][7 + S] = t[f];
     }
-    if (T.hasH && it + 2 < Nz) T.lvl(a, hv[2], it + 2, hcolb);
+    if (T.hasH && it + 2 < Nz) T.lvl(a, hv[S + 2], it + 2, hcolb);
 
     /* ---- P1: plane, rho^-1 and p' of level it ---- */
-    ri[2] = frcp(win[0][5]);
+    ri[S + 2] = frcp(win[0][5 + S]);
     if (z1 < Nz) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) raw[b1][f][ty + HH][tx + HH] = win[f][5];
+      for (int f = 0; f < NF; ++f) raw[b1][f][ty + HH][tx + HH] = win[f][5 + S];
       if (T.hasH) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) (&raw[b1][f][0][0])[T.hslot] = hv[0][f];
+        for (int f = 0; f < NF; ++f) (&raw[b1][f][0][0])[T.hslot] = hv[S][f];
       }
-      const float q = C::NM > 0 ? win[C::IQV][5] : 0.f;
-      p4[3] = pprime(a, win[4][5], T.mfac(a, win[0][5], q), z1);
-      if (it == 0) p4[2] = p4[3];                    /* p'(-1) = p'(0) */
-      (&dvr[b1][0][0])[T.od] = ri[2];
-      (&dvp[b1][0][0])[T.od] = p4[3];
+      const float q = C::NM > 0 ? win[C::IQV][5 + S] : 0.f;
+      pp[S + 3] = pprime(a, win[4][5 + S], T.mfac(a, win[0][5 + S], q), z1);
+      if (it == 0) pp[S + 2] = pp[S + 3];            /* p'(-1) = p'(0) */
+      (&dvr[b1][0][0])[T.od] = ri[S + 2];
+      (&dvp[b1][0][0])[T.od] = pp[S + 3];
       if (T.hasR1) {
-        const float qh = C::NM > 0 ? hv[0][C::IQV] : 0.f;
-        (&dvr[b1][0][0])[T.hd] = frcp(hv[0][0]);
-        (&dvp[b1][0][0])[T.hd] = pprime(a, hv[0][4], T.mfac(a, hv[0][0], qh), z1);
+        const float qh = C::NM > 0 ? hv[S][C::IQV] : 0.f;
+        (&dvr[b1][0][0])[T.hd] = frcp(hv[S][0]);
+        (&dvp[b1][0][0])[T.hd] = pprime(a, hv[S][4], T.mfac(a, hv[S][0], qh), z1);
       }
     } else {
-      p4[3] = p4[2];                                 /* p'(Nz) = p'(Nz-1) */
+      pp[S + 3] = pp[S + 2];                         /* p'(Nz) = p'(Nz-1) */
     }
 
     /* ---- P2: x and y face fluxes of level it-1 ---- */
     if (z2 >= 0 && z2 < Nz) {
       float (*rw)[RH][RW] = raw[b2];
       const float (*dr)[DW] = dvr[b2];
-      pgn[0] = 0.5f * a.dXi * (dvp[b2][ty + 1][tx + 2] - dvp[b2][ty + 1][tx]);
-      pgn[1] = 0.5f * a.dYi * (dvp[b2][ty + 2][tx + 1] - dvp[b2][ty][tx + 1]);
+      pg[S + 1][0] = 0.5f * a.dXi * (dvp[b2][ty + 1][tx + 2] - dvp[b2][ty + 1][tx]);
+      pg[S + 1][1] = 0.5f * a.dYi * (dvp[b2][ty + 2][tx + 1] - dvp[b2][ty][tx + 1]);
       const float Dz = prof(a, PR_D, z2);
       const float hxD = 0.5f * a.dXi * Dz, hyD = 0.5f * a.dYi * Dz;
       {
@@ -1190,35 +1198,48 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
 
     /* ---- P3: own cell of level it-2 ---- */
     if (z3 >= 0) {
-      float dxy[NF], gzhi[NF], F[NF];
+      float dxy[NF], F[NF];
+      float (*wv)[9] = win;
+      float w7[NF][7];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) w7[f][j] = wv[f][j + S];
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         dxy[f] = (gx[b1][f][ty][tx + 1] - gx[b1][f][ty][tx]) + (gy[b1][f][ty + 1][tx] - gy[b1][f][ty][tx]);
-      dyn_tendency<C, 8>(a, z3, win, ri[0], ri[1], p4[0], p4[2], pgc[0], pgc[1], dxy, gzlo, gzhi, T.omb, F);
+      dyn_tendency<C, 7>(a, z3, w7, ri[S], ri[S + 1], pp[S], pp[S + 2], pg[S][0], pg[S][1], dxy, gz[S], gz[S + 1],
+                         T.omb, F);
       if (T.active) {
         float b0[NF];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3] : bse[f];
+        for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3 + S] : bs[S][f];
         rk_store<C>(a, T.fout, o3, F, b0);
-        if (a.pout) *(float *)((char *)a.pout + o3) = p4[1];
+        if (a.pout) *(float *)((char *)a.pout + o3) = pp[S + 1];
       }
+    } else {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) gzlo[f] = gzhi[f];
+      for (int f = 0; f < NF; ++f) gz[S + 1][f] = gz[S][f];
     }
     __syncthreads();
+  };
 
-    /* ---- advance the pipelines ---- */
+  for (int it = 0; it < Nz + 2; it += 2) {
+    step(std::integral_constant<int, 0>{}, it);
+    if (it + 1 < Nz + 2) step(std::integral_constant<int, 1>{}, it + 1);
+    /* ---- rotate the carried arrays by two slots ---- */
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
 #pragma unroll
-      for (int j = 0; j < 7; ++j) win[f][j] = win[f][j + 1];
-      hv[0][f] = hv[1][f];
-      hv[1][f] = hv[2][f];
-      bse[f] = bsn[f];
+      for (int j = 0; j < 7; ++j) win[f][j] = win[f][j + 2];
+      hv[0][f] = hv[2][f];
+      hv[1][f] = hv[3][f];
+      bs[0][f] = bs[2][f];
+      gz[0][f] = gz[2][f];
     }
-    p4[0] = p4[1]; p4[1] = p4[2]; p4[2] = p4[3];
-    ri[0] = ri[1]; ri[1] = ri[2];
-    pgc[0] = pgn[0]; pgc[1] = pgn[1];
+    pp[0] = pp[2]; pp[1] = pp[3]; pp[2] = pp[4];
+    ri[0] = ri[2]; ri[1] = ri[3];
+    pg[0][0] = pg[2][0]; pg[0][1] = pg[2][1];
   }
 }
 

@@ -54,6 +54,9 @@ struct Dev {
   float *prof = nullptr;
   float *S[3] = {nullptr, nullptr, nullptr};
   float *pout = nullptr;
+  /* batch-end emissions (allocated when the host asks for them): SGS stresses
+   * (hydroSubGridWrite), SGS moisture fluxes, tendencies (hydroForcingWrite/Log) */
+  float *tau = nullptr, *mtau = nullptr, *fem = nullptr;
   float *surf = nullptr;           /* 15 planes of [Ny][P] */
   float *stg = nullptr;            /* halo-inclusive staging, one 3-D field */
   long n3h = 0, n2h = 0;
@@ -94,7 +97,8 @@ __global__ void k_host_to_dev(const float *src, float *dst, int Nx, int Ny, int 
   dst[((long)z * Ny + y) * P + x] = src[((long)(x + Nh) * NYH + (y + Nh)) * NZH + (z + Nh)];
 }
 
-/* rule: 0 rho, 1 rho*u, 2 rho*v, 3 rho*w, 4 rho*theta, 5 sgs tke, 6 moisture, 7 pressure */
+/* rule: 0 rho, 1 rho*u, 2 rho*v, 3 rho*w, 4 rho*theta, 5 sgs tke, 6 moisture (and other fields with
+ * zero-gradient halos), 7 pressure, 8 interior only (zero halos) */
 __global__ void k_dev_to_host(const float *src, float *dst, int Nx, int Ny, int Nz, int Nh, int P, int rule,
                               const float *prof, int np, float uTop, float vTop) {
   const int NZH = Nz + 2 * Nh;
@@ -106,7 +110,8 @@ __global__ void k_dev_to_host(const float *src, float *dst, int Nx, int Ny, int 
   y = ((y % Ny) + Ny) % Ny;
   const long col = (long)y * P + x, ps = (long)Ny * P;
   float v;
-  if (z >= 0 && z < Nz) v = src[z * ps + col];
+  if (rule == 8 && (i < Nh || i >= Nx + Nh || j < Nh || j >= Ny + Nh || z < 0 || z >= Nz)) v = 0.f;
+  else if (z >= 0 && z < Nz) v = src[z * ps + col];
   else if (rule == 0) v = prof[PR_RHO * np + z + 3];
   else if (rule == 4) v = prof[PR_RTH * np + z + 3];
   else if (rule == 7) v = (k == 0) ? 0.f : src[(z < 0 ? 0 : Nz - 1) * ps + col];
@@ -136,7 +141,7 @@ typedef void (*stage_fn)(StepArgs);
 typedef void (*surface_fn)(StepArgs);
 
 struct Variant {
-  stage_fn plain, phys;
+  stage_fn plain, phys, plain_emit, phys_emit;   /* *_emit: the batch's last stage, with emissions */
   surface_fn surface;
   const char *name;
 };
@@ -144,8 +149,10 @@ struct Variant {
 template <bool TKE, int NM, int SH, int SE, int SQV, int SQL>
 Variant make_variant(const char *name) {
   Variant v;
-  v.plain = stage_plain_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>>;
-  v.phys = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
+  v.plain = stage_plain_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>, false>;
+  v.plain_emit = stage_plain_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, false>, true>;
+  v.phys = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>, false>;
+  v.phys_emit = stage_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>, true>;
   v.surface = surface_kernel<Cfg<TKE, NM, SH, SE, SQV, SQL, true>>;
   v.name = name;
   return v;
@@ -179,6 +186,9 @@ int select_variant(void) {
 void fill_args(StepArgs &a, int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
   memset(&a, 0, sizeof a);
   a.in = in; a.base = base; a.out = out; a.prof = D.prof; a.pout = pout;
+  if (pout) {                      /* the last stage of a batch also emits what the host will read */
+    a.tau = D.tau; a.mtau = D.mtau; a.fem = D.fem;
+  }
   a.s.cd = surfp(SF_CD); a.s.ch = surfp(SF_CH); a.s.cq = surfp(SF_CQ); a.s.ust = surfp(SF_UST); a.s.hf = surfp(SF_HF);
   a.s.tsk = surfp(SF_TSK); a.s.iol = surfp(SF_IOL); a.s.z0m = surfp(SF_Z0M); a.s.z0t = surfp(SF_Z0T); a.s.qf = surfp(SF_QF);
   a.s.qsk = surfp(SF_QSK); a.s.t13 = surfp(SF_T13); a.s.t23 = surfp(SF_T23); a.s.th3 = surfp(SF_TH3); a.s.tq3 = surfp(SF_TQ3);
@@ -199,6 +209,7 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
   a.uTop = D.uTop; a.vTop = D.vTop;
   a.c_k = c_k; a.kappa = kappa; a.refP = refPressure; a.presg = pres_grnd; a.R_cp = R_cp; a.L_v = L_v; a.cp = cp_gas;
   a.R_vapor = R_vapor; a.condTs = moistureCondTscale; a.condSel = moistureCond; a.condBase = moistureCondBasePres;
+  a.mSGS = moistureSGSturb > 0;
   a.step_it = simTime_it; a.step_itR = simTime_itRestart; a.simTime = simTime; a.dt = dt;
   a.slSel = surflayerSelector; a.slIdeal = surflayer_idealsine; a.slStab = surflayer_stab;
   a.slWth = surflayer_wth; a.slWq = surflayer_wq; a.slTr = surflayer_tr; a.slQr = surflayer_qr;
@@ -227,7 +238,10 @@ int launch_stage(int stage, const float *in, const float *base, float *out, int 
   }
   dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY);
   timed_begin(phys ? 2 : 1);
-  hipLaunchKernelGGL(phys ? g_var.phys : g_var.plain, grid, dim3(NT), 0, D.stream, a);
+  /* the batch's last stage also emits the SGS stresses and tendencies the host will read */
+  const bool emit = a.tau || a.mtau || a.fem;
+  stage_fn k = phys ? (emit ? g_var.phys_emit : g_var.phys) : (emit ? g_var.plain_emit : g_var.plain);
+  hipLaunchKernelGGL(k, grid, dim3(NT), 0, D.stream, a);
   timed_end();
   HIPCHK(hipGetLastError());
   return FE_OK;
@@ -317,6 +331,22 @@ int download_state(int with_pres) {
     if ((rc = download_field(field_ptr(D.S[0], 5 + D.nTKE + f), moistScalars + f * n3, 6))) return rc;
   if (with_pres && hydroPres)
     if ((rc = download_field(D.pout, hydroPres, 7))) return rc;
+  /* batch-end emissions (cuda_timeIntDevice.cu:210-247): tendencies with zero halos,
+   * SGS stresses and moisture fluxes with periodic and zero-gradient halos */
+  if (with_pres && D.fem) {
+    for (int f = 0; f < 5; ++f)
+      if ((rc = download_field(D.fem + (long)f * D.fs, hydroFldsFrhs + f * n3, 8))) return rc;
+    for (int f = 0; f < D.nTKE; ++f)
+      if ((rc = download_field(D.fem + (long)(5 + f) * D.fs, sgstkeScalarsFrhs + f * n3, 8))) return rc;
+    for (int f = 0; f < D.nM; ++f)
+      if ((rc = download_field(D.fem + (long)(5 + D.nTKE + f) * D.fs, moistScalarsFrhs + f * n3, 8))) return rc;
+  }
+  if (with_pres && D.tau)
+    for (int f = 0; f < 9; ++f)
+      if ((rc = download_field(D.tau + (long)f * D.fs, hydroTauFlds + f * n3, 6))) return rc;
+  if (with_pres && D.mtau)
+    for (int f = 0; f < 3 * D.nM; ++f)
+      if ((rc = download_field(D.mtau + (long)f * D.fs, moistTauFlds + f * n3, 6))) return rc;
   if (surflayerSelector > 0) {
     if ((rc = download_2d(surfp(SF_UST), fricVel))) return rc;
     if ((rc = download_2d(surfp(SF_HF), htFlux))) return rc;
@@ -485,6 +515,18 @@ int cuda_hydroCoreInit(void) {
   }
   HIPCHK(hipMalloc(&D.pout, (size_t)D.fs * sizeof(float)));
   HIPCHK(hipMemset(D.pout, 0, (size_t)D.fs * sizeof(float)));
+  if (hydroSubGridWrite == 1 && turbulenceSelector > 0 && hydroTauFlds) {
+    HIPCHK(hipMalloc(&D.tau, (size_t)9 * D.fs * sizeof(float)));
+    HIPCHK(hipMemset(D.tau, 0, (size_t)9 * D.fs * sizeof(float)));
+  }
+  if (hydroSubGridWrite == 1 && moistureSGSturb == 1 && D.nM > 0 && moistTauFlds) {
+    HIPCHK(hipMalloc(&D.mtau, (size_t)3 * D.nM * D.fs * sizeof(float)));
+    HIPCHK(hipMemset(D.mtau, 0, (size_t)3 * D.nM * D.fs * sizeof(float)));
+  }
+  if (hydroForcingWrite == 1 || hydroForcingLog == 1) {
+    HIPCHK(hipMalloc(&D.fem, (size_t)D.NF * D.fs * sizeof(float)));
+    HIPCHK(hipMemset(D.fem, 0, (size_t)D.NF * D.fs * sizeof(float)));
+  }
   HIPCHK(hipMalloc(&D.surf, (size_t)SF_N * D.ps * sizeof(float)));
   HIPCHK(hipMemset(D.surf, 0, (size_t)SF_N * D.ps * sizeof(float)));
   HIPCHK(hipMalloc(&D.stg, (size_t)D.n3h * sizeof(float)));
@@ -494,7 +536,7 @@ int cuda_hydroCoreInit(void) {
 
 int cuda_hydroCoreCleanup(void) {
   for (int s = 0; s < 3; ++s) if (D.S[s]) { hipFree(D.S[s]); D.S[s] = nullptr; }
-  float **p[] = {&D.pout, &D.surf, &D.stg};
+  float **p[] = {&D.pout, &D.tau, &D.mtau, &D.fem, &D.surf, &D.stg};
   for (auto q : p) if (*q) { hipFree(*q); *q = nullptr; }
   D.hydro = 0;
   return FE_OK;

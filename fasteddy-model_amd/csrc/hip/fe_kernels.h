@@ -42,6 +42,9 @@ struct StepArgs {
   float *out;           /* stage output (or F in emit mode)         */
   const float *prof;    /* [NPROF][Nz+6]                            */
   float *pout;          /* optional perturbation-pressure emission [Nz][Ny][P] */
+  float *tau;           /* optional SGS stress emission, 9 x [Nz][Ny][P] (11,21,31,32,22,33,th1,th2,th3) */
+  float *mtau;          /* optional SGS moisture-flux emission, 3 x NM x [Nz][Ny][P] */
+  float *fem;           /* optional tendency emission next to the RK update, NF x [Nz][Ny][P] */
   SurfArgs s;
   int Nx, Ny, Nz, P;
   long fs, ps;          /* field stride, plane stride (Ny*P)        */
@@ -57,6 +60,7 @@ struct StepArgs {
   float uTop, vTop;     /* momentum in the top halo (geostrophic base state) */
   float c_k, kappa, refP, presg, R_cp, L_v, cp, R_vapor, condTs;
   int condSel, condBase, mpActive, mpTheta;
+  int mSGS;              /* SGS moisture fluxes on (moistureSGSturb) */
   int step_it, step_itR;
   float simTime, dt;
   int slSel, slIdeal, slStab;

@@ -387,9 +387,13 @@ __device__ __forceinline__ void dyn_tendency(const StepArgs &a, int z, float (*w
 
 /* RK3-WS2002 stage update of the own cell (cuda_RKschemes.cu:113-157, positivity of
  * rho*e and moisture), or the tendency itself in emit mode. */
-template <class C>
+template <class C, bool EMIT>
 __device__ __forceinline__ void rk_store(const StepArgs &a, char *const *fout, unsigned oz, const float *F,
                                          const float *base) {
+  if (EMIT && a.fem) {
+#pragma unroll
+    for (int f = 0; f < C::NF; ++f) *(float *)((char *)(a.fem + (long)f * a.fs) + oz) = F[f];
+  }
   if (a.emitF) {
 #pragma unroll
     for (int f = 0; f < C::NF; ++f) *(float *)(fout[f] + oz) = F[f];
@@ -546,7 +550,7 @@ struct Tile {
  * registers), so one copy suffices; velocity planes read in P3 and written in
  * the next P1 rotate over four levels (u, v, w: levels z-1..z+1 are live) or
  * three (theta and moisture: z, z+1). */
-template <class C>
+template <class C, bool EMIT = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS ? 2 : FE_PLAIN_WPE, 8))) stage_kernel(StepArgs a) {
   constexpr int NF = C::NF;
   constexpr bool PH = C::PHYS;
@@ -935,10 +939,12 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         F[2] -= a.dXi * (tb1[1][ty][tx + 1] - tb1[1][ty][tx]) + a.dYi * (tb2[1][ty + 1][tx] - tb2[1][ty][tx]) + zdz * (zn[1] - zc[1]);
         F[3] -= a.dXi * (tb1[2][ty][tx + 1] - tb1[2][ty][tx]) + a.dYi * (tb2[2][ty + 1][tx] - tb2[2][ty][tx]) + zdz * (zn[2] - zc[2]);
         F[4] -= a.dXi * (tb1[3][ty][tx + 1] - tb1[3][ty][tx]) + a.dYi * (tb2[3][ty + 1][tx] - tb2[3][ty][tx]) + zdz * (zn[3] - zc[3]);
+        if (a.mSGS) {
 #pragma unroll
-        for (int m = 0; m < C::NM; ++m)
-          F[C::IQV + m] -= a.dXi * (tb1[4 + m][ty][tx + 1] - tb1[4 + m][ty][tx]) + a.dYi * (tb2[4 + m][ty + 1][tx] - tb2[4 + m][ty][tx])
-                          + zdz * (zn[4 + m] - zc[4 + m]);
+          for (int m = 0; m < C::NM; ++m)
+            F[C::IQV + m] -= a.dXi * (tb1[4 + m][ty][tx + 1] - tb1[4 + m][ty][tx]) + a.dYi * (tb2[4 + m][ty + 1][tx] - tb2[4 + m][ty][tx])
+                            + zdz * (zn[4 + m] - zc[4 + m]);
+        }
         if constexpr (C::TKE) {
           const float rhoe = win[C::IE][3];
           const float e = rhoe * rin;
@@ -1003,6 +1009,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             F[4] += ((thd * a.L_v) / (Tf * a.cp)) * fc;
           }
         }
+        if (EMIT && active && a.tau) {
+          /* SGS stresses of the own cell at level z, in the reference's field order
+           * (11, 21, 31, 32, 22, 33, theta 1..3; moisture x, y, z per species) */
+          const float tv[9] = {t11, t21, zc[0], zc[1], t22, zc[2], tb1[3][ty][tx], tb2[3][ty][tx], zc[3]};
+#pragma unroll
+          for (int k = 0; k < 9; ++k) *(float *)((char *)(a.tau + (long)k * a.fs) + oz) = tv[k];
+          if (C::NM > 0 && a.mtau && a.mSGS) {
+#pragma unroll
+            for (int m = 0; m < C::NM; ++m) {
+              *(float *)((char *)(a.mtau + (long)(3 * m) * a.fs) + oz) = tb1[4 + m][ty][tx];
+              *(float *)((char *)(a.mtau + (long)(3 * m + 1) * a.fs) + oz) = tb2[4 + m][ty][tx];
+              *(float *)((char *)(a.mtau + (long)(3 * m + 2) * a.fs) + oz) = zc[4 + m];
+            }
+          }
+        }
 #pragma unroll
         for (int q = 0; q < 4 + C::NM; ++q) zc[q] = zn[q];
         Aown[0] = Aown[1];
@@ -1013,7 +1034,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float b0[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) b0[f] = baseIn ? win[f][3] : bse[f];
-        rk_store<C>(a, fout, oz, F, b0);
+        rk_store<C, EMIT>(a, fout, oz, F, b0);
         if (a.pout) *(float *)((char *)a.pout + oz) = pz[1];
       }
 #pragma unroll
@@ -1078,7 +1099,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
  * load is issued one iteration before the iteration that first uses it; the own
  * column and the halo cell are both loaded two levels ahead of P1, so a tile fetches
  * its halo lines while its neighbours fetch the same lines as their own columns. */
-template <class C>
+template <class C, bool EMIT = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAIN_WPE, 8))) stage_plain_kernel(StepArgs a) {
   constexpr int NF = C::NF;
   __shared__ float raw[2][NF][RH][RW];
@@ -1214,7 +1235,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
         float b0[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3 + S] : bs[S][f];
-        rk_store<C>(a, T.fout, o3, F, b0);
+        rk_store<C, EMIT>(a, T.fout, o3, F, b0);
         if (a.pout) *(float *)((char *)a.pout + o3) = pp[S + 1];
       }
     } else {

@@ -297,3 +297,43 @@ def test_long_run_profiles_dry_cbl(tmp_path):
         o32.close()
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("moist", [False, True])
+def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
+    """hydroSubGridWrite = 1 and hydroForcingWrite = 1: after a cuda_timeIntCommence batch the host
+    holds the SGS stresses (hydroTauFlds, moistTauFlds) and the tendencies (hydroFldsFrhs, ...)
+    of the batch's last stage, as the reference's device-to-host copy leaves them
+    (cuda_timeIntDevice.cu:198-247); compared with the oracle stepped the same way."""
+    if moist:
+        p = dict(fe_py.CFG2, Nx=32, Ny=16, Nz=16, d_zeta=80.0)
+    else:
+        p = dict(fe_py.CFG0, Nx=64, Ny=16, Nz=24)
+    p.update(NtBatch=2, hydroSubGridWrite=1, hydroForcingWrite=1)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=31)
+        if moist:
+            fe_py.moist_state(m, seed=32)
+        o32, o64 = oracle_py.Oracle(p, 32), oracle_py.Oracle(p, 64)
+        o32.load_from_model(m)
+        o64.load_from_model(m)
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        m.commence()
+        o32.steps(2)
+        o64.steps(2)
+        pairs = [("hydroTauFlds", "TAU", 9), ("hydroFldsFrhs", "HYDRO_F", 5), ("sgstkeScalarsFrhs", "TKE_F", 1)]
+        if moist:
+            pairs += [("moistTauFlds", "MOISTTAU", 6), ("moistScalarsFrhs", "MOIST_F", 2)]
+        for hname, oname, nf in pairs:
+            g = m.interior(m.array(hname)).astype(np.float64)[:nf]
+            r64 = m.interior(o64.arr(oname)[:nf])
+            r32 = m.interior(o32.arr(oname)[:nf]).astype(np.float64)
+            e_gpu, e_ref = rel_err(g, r64), rel_err(r32, r64)
+            print(f"emitted {hname}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
+            assert np.all(np.isfinite(g))
+            assert np.all(e_gpu <= np.maximum(1e-4, 2 * e_ref)), (hname, e_gpu, e_ref)
+        o32.close()
+        o64.close()
+    finally:
+        m.close()

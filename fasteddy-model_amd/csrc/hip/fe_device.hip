@@ -669,10 +669,12 @@ double fe_copy_probe(long nfloats, int reps) {
  * partial sums meet in LDS.  Output columns, per level:
  *   0 <u> 1 <v> 2 <w> 3 <theta> 4 <u'u'> 5 <u'w'> 6 <v'v'> 7 <v'w'> 8 <w'w'>
  *   9 resolved TKE 10 <theta'w'> 11 <e> 12 <q_v> 13 <q_l>
+ *   14 (<tau13>^2 + <tau23>^2)^1/2, 15 <tau33>, 16 <tau_theta3> (level means of the SGS
+ *   stresses emitted by the last batch; moved to cell centres on the host)
  * (covariances from FP64 first and second moments). */
-constexpr int PROF_NSTAT = 14, PROF_NSUM = 13, PROF_NT = 256;
+constexpr int PROF_NSTAT = 17, PROF_NSUM = 17, PROF_NT = 256;
 __global__ void __launch_bounds__(PROF_NT) k_profiles(const float *S, long fs, long ps, int Nx, int Ny, int P, int nTKE,
-                                                      int nM, double *out) {
+                                                      int nM, const float *tau, double *out) {
   const int z = blockIdx.x;
   double acc[PROF_NSUM];
 #pragma unroll
@@ -688,6 +690,12 @@ __global__ void __launch_bounds__(PROF_NT) k_profiles(const float *S, long fs, l
     if (nTKE > 0) acc[10] += S[5 * fs + o] / rho;
     if (nM > 0) acc[11] += S[(long)(5 + nTKE) * fs + o] / rho;
     if (nM > 1) acc[12] += S[(long)(6 + nTKE) * fs + o] / rho;
+    if (tau) {
+      acc[13] += tau[2 * fs + o];
+      acc[14] += tau[3 * fs + o];
+      acc[15] += tau[5 * fs + o];
+      acc[16] += tau[8 * fs + o];
+    }
   }
 #pragma unroll
   for (int k = 0; k < PROF_NSUM; ++k)
@@ -716,6 +724,9 @@ __global__ void __launch_bounds__(PROF_NT) k_profiles(const float *S, long fs, l
     r[9] = 0.5 * (r[4] + r[6] + r[8]);
     r[10] = s[9] - s[3] * s[2];
     r[11] = s[10]; r[12] = s[11]; r[13] = s[12];
+    r[14] = sqrt(s[13] * s[13] + s[14] * s[14]);
+    r[15] = s[15];
+    r[16] = s[16];
   }
 }
 
@@ -724,12 +735,18 @@ int fe_profiles(double *out, int nstat) {
   if (nstat != PROF_NSTAT || !out) return FE_ERR_UNSUPPORTED;
   double *d = nullptr;
   HIPCHK(hipMalloc(&d, (size_t)D.Nz * PROF_NSTAT * sizeof(double)));
-  hipLaunchKernelGGL(k_profiles, dim3(D.Nz), dim3(PROF_NT), 0, D.stream, D.S[0], D.fs, D.ps, D.Nx, D.Ny, D.P, D.nTKE, D.nM, d);
+  hipLaunchKernelGGL(k_profiles, dim3(D.Nz), dim3(PROF_NT), 0, D.stream, D.S[0], D.fs, D.ps, D.Nx, D.Ny, D.P, D.nTKE, D.nM,
+                     D.tau, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)D.Nz * PROF_NSTAT * sizeof(double), hipMemcpyDeviceToHost, D.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
   hipFree(d);
   HIPCHK(e);
+  /* SGS columns on cell centres, as fetutorialfunctions.py:426-429 (the top level stays 0) */
+  for (int c = 14; c < 17; ++c) {
+    for (int k = 0; k + 1 < D.Nz; ++k) out[k * PROF_NSTAT + c] = 0.5 * (out[k * PROF_NSTAT + c] + out[(k + 1) * PROF_NSTAT + c]);
+    out[(D.Nz - 1) * PROF_NSTAT + c] = 0.0;
+  }
   return FE_OK;
 }
 

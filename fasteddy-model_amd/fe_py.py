@@ -193,10 +193,11 @@ class Model:
 
     def profiles(self):
         """Horizontal-mean profile statistics of the device state, reduced on the device in
-        FP64 (fe_profiles): [Nz][14], columns <u>, <v>, <w>, <theta>, <u'u'>, <u'w'>, <v'v'>,
-        <v'w'>, <w'w'>, resolved TKE, <theta'w'>, <e>, <q_v>, <q_l>."""
-        out = np.zeros((self.geti("Nzp"), 14))
-        rc = self.L.fe_profiles(out.ctypes.data_as(C.POINTER(C.c_double)), 14)
+        FP64 (fe_profiles): [Nz][17], columns <u>, <v>, <w>, <theta>, <u'u'>, <u'w'>, <v'v'>,
+        <v'w'>, <w'w'>, resolved TKE, <theta'w'>, <e>, <q_v>, <q_l>, and the SGS stress columns
+        (|<tau13>, <tau23>|, <tau33>, <tau_theta3> on cell centres; 0 unless stresses are emitted)."""
+        out = np.zeros((self.geti("Nzp"), 17))
+        rc = self.L.fe_profiles(out.ctypes.data_as(C.POINTER(C.c_double)), 17)
         if rc != 0:
             raise RuntimeError(f"fe_profiles failed with {rc}")
         return out

@@ -77,8 +77,10 @@ double fe_copy_probe(long nfloats, int reps);
 /* Name of the kernel variant selected for the stage-1/2 and stage-3 tendencies. */
 const char *fe_kernel_info(void);
 /* Horizontal-mean profile statistics of the current device state, computed on the
- * device in FP64: out[Nz][nstat] with nstat = 14 columns <u>, <v>, <w>, <theta>,
- * <u'u'>, <u'w'>, <v'v'>, <v'w'>, <w'w'>, resolved TKE, <theta'w'>, <e>, <q_v>, <q_l>.
+ * device in FP64: out[Nz][nstat] with nstat = 17 columns <u>, <v>, <w>, <theta>,
+ * <u'u'>, <u'w'>, <v'v'>, <v'w'>, <w'w'>, resolved TKE, <theta'w'>, <e>, <q_v>, <q_l>,
+ * and, from the SGS stresses emitted by the last batch (hydroSubGridWrite), the
+ * cell-centred |<tau13>, <tau23>|, <tau33>, <tau_theta3> (0 otherwise).
  * No counterpart in the reference's C code; it restates the reference's
  * post-processing (tutorials/notebooks/fetutorialfunctions.py:55-83,344-431). */
 int fe_profiles(double *out, int nstat);

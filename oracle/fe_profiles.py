@@ -5,25 +5,27 @@ TEST INFRASTRUCTURE ONLY: the checker for the device reduction
 post-processing of an output state, tutorials/notebooks/fetutorialfunctions.py:
 compute_mean_profiles (:55-83: level means of u, v, theta) and
 compute_turb_profiles (:344-431: perturbations from the instantaneous level
-means, their covariances, resolved TKE, the mean SGS TKE).  The SGS-stress
-columns of :406-429 need stored stress fields, which the fused time step does
-not keep, and are not part of this set.  Velocities, theta, SGS TKE and
+means, their covariances, resolved TKE, the mean SGS TKE, and the SGS stress
+columns of :406-429 from stress fields emitted at the end of a batch).  Velocities, theta, SGS TKE and
 moisture are rho-weighted state divided by rho, as the reference's writers
 store them (SRC/IO/io_binary.c:55-97).
 
-Columns (NSTAT = 14), one row per level:
+Columns (NSTAT = 17), one row per level:
   0 <u>  1 <v>  2 <w>  3 <theta>
   4 <u'u'>  5 <u'w'>  6 <v'v'>  7 <v'w'>  8 <w'w'>  9 resolved TKE 1/2(<u'u'>+<v'v'>+<w'w'>)
   10 <theta'w'>  11 <e> (SGS TKE)  12 <q_v>  13 <q_l>  (moisture in g/kg; 0 when absent)
+  14 (<tau13>^2 + <tau23>^2)^1/2  15 <tau33>  16 <tau_theta3>, each averaged from the level
+     to the cell centre above it (:426-429; the top level is 0; 0 without stresses)
 """
 import numpy as np
 
-NSTAT = 14
+NSTAT = 17
 
 
-def profiles(hydro, tke=None, moist=None, nh=3):
+def profiles(hydro, tke=None, moist=None, nh=3, tau=None):
     """hydro: [5][Nx+2nh][Ny+2nh][Nz+2nh] rho-weighted state (halo-inclusive, z fastest);
-    tke, moist: [n][...] or None.  Returns float64 [Nz][NSTAT]."""
+    tke, moist: [n][...] or None; tau: the 9 SGS stress fields in the reference's order
+    (11, 21, 31, 32, 22, 33, theta 1..3) or None.  Returns float64 [Nz][NSTAT]."""
     sl = (slice(nh, -nh), slice(nh, -nh), slice(nh, -nh))
     rho = hydro[0][sl].astype(np.float64)
     u, v, w, th = (hydro[f][sl].astype(np.float64) / rho for f in (1, 2, 3, 4))
@@ -46,4 +48,8 @@ def profiles(hydro, tke=None, moist=None, nh=3):
         out[:, 12] = mean(moist[0][sl].astype(np.float64) / rho)
         if moist.shape[0] > 1:
             out[:, 13] = mean(moist[1][sl].astype(np.float64) / rho)
+    if tau is not None:
+        t13, t23, t33, tt3 = (mean(tau[k][sl].astype(np.float64)) for k in (2, 3, 5, 8))
+        for c, col in ((14, np.sqrt(t13 ** 2 + t23 ** 2)), (15, t33), (16, tt3)):
+            out[:-1, c] = 0.5 * (col[:-1] + col[1:])
     return out

@@ -333,6 +333,15 @@ def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
             print(f"emitted {hname}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
             assert np.all(np.isfinite(g))
             assert np.all(e_gpu <= np.maximum(1e-4, 2 * e_ref)), (hname, e_gpu, e_ref)
+        # profile statistics with the SGS stress columns: device reduction against the numpy
+        # restatement over the same downloaded state and emitted stresses
+        import fe_profiles
+        dev = m.profiles()
+        ref = fe_profiles.profiles(m.array("hydroFlds"), m.array("sgstkeScalars"), m.array("moistScalars"),
+                                   tau=m.array("hydroTauFlds"))
+        assert np.any(ref[:, 14:] != 0.0)
+        scale = np.maximum(np.abs(ref).max(axis=0), 1e-30)
+        assert np.all((np.abs(dev - ref) / scale).max(axis=0) < 1e-10)
         o32.close()
         o64.close()
     finally:

@@ -33,8 +33,22 @@ def test_profile_statistics_of_sinusoids():
         tke[0][sl + (k,)] = rho * E
         moist[0][sl + (k,)] = rho * 12.0
         moist[1][sl + (k,)] = rho * 0.5
-    p = fe_profiles.profiles(h, tke, moist, nh=NH)
+    # SGS stresses: level means tau13 = 3k, tau23 = 4k (magnitude 5k), tau33 = k, tau_theta3 = -k
+    # with zero-mean fluctuations; on cell centres the columns are 5(k+1/2), k+1/2, -(k+1/2)
+    tau = np.zeros((9,) + h.shape[1:])
+    for k in range(nz):
+        kk = NH + k
+        tau[2][sl + (kk,)] = 3.0 * k + sx
+        tau[3][sl + (kk,)] = 4.0 * k - cy
+        tau[5][sl + (kk,)] = 1.0 * k + sx * cy
+        tau[8][sl + (kk,)] = -1.0 * k
+    p = fe_profiles.profiles(h, tke, moist, nh=NH, tau=tau)
     expect = [U0, V0, 0.0, 300.0, A * A / 2, A * B / 2, Cc * Cc / 2, 0.0, B * B / 2,
               (A * A + B * B + Cc * Cc) / 4, D * B / 2, E, 12.0, 0.5]
     assert p.shape == (nz, fe_profiles.NSTAT)
-    np.testing.assert_allclose(p, np.tile(expect, (nz, 1)), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(p[:, :14], np.tile(expect, (nz, 1)), rtol=1e-12, atol=1e-12)
+    kc = np.arange(nz - 1) + 0.5
+    np.testing.assert_allclose(p[:-1, 14], 5.0 * kc, rtol=1e-12)
+    np.testing.assert_allclose(p[:-1, 15], kc, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(p[:-1, 16], -kc, rtol=1e-12)
+    assert np.all(p[-1, 14:] == 0.0)

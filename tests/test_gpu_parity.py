@@ -346,3 +346,41 @@ def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
         o64.close()
     finally:
         m.close()
+
+
+VARIANTS = [
+    ("dry+tke weno5/weno5", dict(advectionSelector=5, TKEAdvSelector=5), False),
+    ("dry+tke hyb34/hyb34", dict(advectionSelector=2, TKEAdvSelector=2, b_hyb=0.3, TKEAdvSelector_b_hyb=0.6), False),
+    ("dry+tke quick/quick", dict(advectionSelector=1, TKEAdvSelector=1), False),
+    ("dry+tke upstream", dict(advectionSelector=0, TKEAdvSelector=0), False),
+    ("dry+tke centred2", dict(advectionSelector=6, TKEAdvSelector=6), False),
+    ("dry+tke weno3", dict(advectionSelector=4, TKEAdvSelector=4), False),
+    ("dry+tke hyb56/upstream", dict(advectionSelector=3, TKEAdvSelector=0, b_hyb=0.5), False),
+    ("dry inviscid hyb56", dict(turbulenceSelector=0, TKESelector=0, surflayerSelector=0), False),
+    ("moist2+tke hyb56/.../upstream", dict(moistureAdvSelectorQi=0), True),
+    ("moist1+tke hyb56", dict(moistureNvars=1), True),
+]
+
+
+@pytest.mark.parametrize("name,over,moist", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_stage_tendencies_every_variant(name, over, moist, tmp_path):
+    """Every compiled kernel variant (advection scheme per field family, TKE on/off, one or two
+    moisture species) against the oracle on a grid with partial tiles in x and y."""
+    base = dict(fe_py.CFG2, Nx=40, Ny=12, Nz=14, d_zeta=60.0) if moist else dict(fe_py.CFG0, Nx=40, Ny=12, Nz=14)
+    p = dict(base, **over)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        assert m.L.fe_kernel_info().decode() == name
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=len(name))
+        if moist:
+            fe_py.moist_state(m, seed=len(name) + 1)
+        m.upload()
+        o32, o64 = oracle_py.Oracle(p, 32), oracle_py.Oracle(p, 64)
+        o32.load_from_model(m)
+        o64.load_from_model(m)
+        for stage in (0, 2):
+            check_tendencies(m, o32, o64, stage, name)
+        o32.close()
+        o64.close()
+    finally:
+        m.close()

@@ -536,11 +536,15 @@ struct Tile {
   }
 };
 
-/* One fused RK stage.  Per level z, with two work-group barriers:
+/* The RK stage that carries the SGS physics (the third stage when the physics
+ * runs once per step; every stage otherwise).  The stages without it use
+ * stage_plain_kernel below; this kernel is instantiated with C::PHYS only, and
+ * its !PHYS branches are the same algorithm without the SGS parts.  Per level z,
+ * with two work-group barriers:
  *   P1  write the xy plane of level z (own cell from the register window,
- *       halo from registers prefetched one or two levels earlier), rho^-1 and
- *       p' on the ring-1 region; physics stage: velocities of level z+1 and
- *       rho*Km terms of level z on the ring-1 region;
+ *       halo from registers prefetched two levels earlier), rho^-1 and p' on
+ *       the ring-1 region, velocities of level z+1 and rho*Km terms of level z
+ *       on the ring-1 region;
  *   P2  x and y face fluxes (each face once), SGS stresses on x and y faces,
  *       cell-centred TKE transport fluxes;
  *   P3  own-cell tendency (flux differences, z faces from registers, pressure
@@ -555,11 +559,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   constexpr int NF = C::NF;
   constexpr bool PH = C::PHYS;
   /* Prefetch distances in levels: the own column's level z+OD and the halo's
-   * level z+PD are loaded while level z is computed.  The stage without SGS
-   * physics loads both at z+3, so that a tile's halo lines are fetched at
-   * about the time its neighbours fetch the same lines as their own columns
-   * and are served from L2; the physics stage (register bound) keeps the halo
-   * two levels ahead. */
+   * level z+PD are loaded while level z is computed (the register-bound physics
+   * stage keeps the halo two levels ahead). */
   constexpr int OD = PH ? 4 : 3;
   constexpr int PD = PH ? 2 : 3;
   constexpr int NV = PH ? C::NV : 1;

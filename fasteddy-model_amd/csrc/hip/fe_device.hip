@@ -198,6 +198,7 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
   a.coefdt = coef[stage] * dt;
   a.bh = 1.f - b_hyb; a.be = 1.f - TKEAdvSelector_b_hyb; a.bqv = 1.f - moistureAdvSelectorQv_b;
   a.g = accel_g; a.fH = corioConstHorz; a.fV = corioConstVert;
+  a.pgf = pgfSelector > 0 ? 1.f : 0.f; a.gb = buoyancySelector > 0 ? accel_g : 0.f;
   a.Rv_Rg = Rv_Rg; a.c1p = R_gas / powf(refPressure, R_cp); a.gam = cp_cv;
   {
     double c = 1.0, gm = (double)cp_cv;
@@ -460,7 +461,7 @@ int cuda_hydroCoreInit(void) {
             advectionSelector, TKEAdvSelector, turbulenceSelector, TKESelector, moistureSelector, moistureNvars);
     return FE_ERR_UNSUPPORTED;
   }
-  if (pgfSelector != 1 || buoyancySelector != 1 || hydroBCs != 2) return FE_ERR_UNSUPPORTED;
+  if (hydroBCs != 2) return FE_ERR_UNSUPPORTED;
   D.nTKE = (turbulenceSelector > 0 && TKESelector > 0) ? 1 : 0;
   D.nM = moistureSelector > 0 ? moistureNvars : 0;
   D.NF = 5 + D.nTKE + D.nM;

@@ -52,6 +52,7 @@ struct StepArgs {
   float dXi, dYi, dZi, coefdt;
   float bh, be, bqv;    /* (1-b) for the hybrid schemes of each field group */
   float g, fH, fV;
+  float pgf, gb;         /* 1/0 with pgfSelector; g/0 with buoyancySelector */
   float Rv_Rg, c1p, gam;
   float pc[8];          /* binomial-series coefficients of (1+r)^gam - 1 */
   int pform;

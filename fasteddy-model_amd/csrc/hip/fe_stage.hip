@@ -348,12 +348,12 @@ __device__ __forceinline__ void dyn_tendency(const StepArgs &a, int z, float (*w
   for (int f = 0; f < NF; ++f) F[f] = -invD * (dxy[f] + (gzhi[f] - gzlo[f]));
   const float rho = win[0][3];
   {
-    float fac = 1.f;
+    float fac = a.pgf;
     if constexpr (C::NM > 0) {
       float qs = 0.f;
 #pragma unroll
       for (int m = 0; m < C::NM; ++m) qs += win[C::IQV + m][3];
-      fac = 1.f / (1.f + qs * rin * 1e-3f);
+      fac = a.pgf / (1.f + qs * rin * 1e-3f);
     }
     F[1] -= fac * pgx;
     F[2] -= fac * pgy;
@@ -361,12 +361,12 @@ __device__ __forceinline__ void dyn_tendency(const StepArgs &a, int z, float (*w
   }
   if (a.damp) F[3] -= prof(a, PR_DAMP, z) * win[3][3];
   if constexpr (C::NM == 0) {
-    F[3] -= a.g * (rho - prof(a, PR_RHO, z));
+    F[3] -= a.gb * (rho - prof(a, PR_RHO, z));
   } else {
     float rt = rho;
 #pragma unroll
     for (int m = 0; m < C::NM; ++m) rt += win[C::IQV + m][3] * 1e-3f;
-    F[3] -= (rho / rt) * a.g * (rt - prof(a, PR_RHO, z));
+    F[3] -= (rho / rt) * a.gb * (rt - prof(a, PR_RHO, z));
   }
   {
     float u = win[1][3] * rin, v = win[2][3] * rin, w = win[3][3] * rin;

@@ -384,3 +384,68 @@ def test_stage_tendencies_every_variant(name, over, moist, tmp_path):
         o64.close()
     finally:
         m.close()
+
+
+OPTIONS = [
+    ("skin-temperature surface layer", dict(surflayerSelector=2, surflayer_tr=3.6), False),
+    ("ideal sine surface heat flux", dict(surflayer_idealsine=1, surflayer_ideal_ts=0.0, surflayer_ideal_te=0.2,
+                                           surflayer_ideal_amp=0.3), False),
+    ("neutral surface layer", dict(surflayer_stab=1), False),
+    ("vertical Coriolis", dict(coriolisSelector=2), False),
+    ("physics every stage", dict(physics_oneRKonly=0), False),
+    ("no damping layer", dict(dampingLayerSelector=0), False),
+    ("no pressure gradient or buoyancy", dict(pgfSelector=0, buoyancySelector=0), False),
+    ("moist without pressure gradient or buoyancy", dict(pgfSelector=0, buoyancySelector=0), True),
+    ("moist skin surface layer", dict(surflayerSelector=2, surflayer_tr=3.6, surflayer_qr=1.8), True),
+    ("moist condensation form 1", dict(moistureCond=1, moistureCondBasePres=0), True),
+    ("moist condensation form 2", dict(moistureCond=2), True),
+    ("moist without SGS moisture fluxes", dict(moistureSGSturb=0), True),
+]
+
+
+@pytest.mark.parametrize("name,over,moist", OPTIONS, ids=[o[0] for o in OPTIONS])
+def test_rk3_steps_options(name, over, moist, tmp_path):
+    """Selector and parameter options of the physics through three full RK3 steps against the
+    FP64 oracle: state, surface fields and (where present) moisture."""
+    base = dict(fe_py.CFG2, Nx=32, Ny=16, Nz=16, d_zeta=80.0) if moist else dict(fe_py.CFG0, Nx=48, Ny=16, Nz=20)
+    p = dict(base, NtBatch=1, **over)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=len(name))
+        if moist:
+            fe_py.moist_state(m, seed=len(name) + 1)
+        o64 = oracle_py.Oracle(p, 64)
+        o64.load_from_model(m)
+        o32 = oracle_py.Oracle(p, 32)
+        o32.load_from_model(m)
+        checks = [("hydroFlds", "HYDRO"), ("sgstkeScalars", "TKE")] + ([("moistScalars", "MOIST")] if moist else [])
+        start = {h: m.interior(m.array(h)).astype(np.float64).copy() for h, _ in checks}
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        for _ in range(3):
+            m.commence()
+        o64.steps(3)
+        for hname, oname in checks:
+            g = m.interior(m.array(hname)).astype(np.float64)
+            o = m.interior(o64.arr(oname)[: g.shape[0]])
+            scale = np.abs(o).max(axis=(1, 2, 3))
+            change = np.abs(o - start[hname]).max(axis=(1, 2, 3))
+            diff = np.abs(g - o).max(axis=(1, 2, 3))
+            print(f"{name} {hname}: |gpu-fp64| {diff} vs |change| {change}")
+            assert np.all(diff <= 2e-3 * change + 1e-6 * scale), (hname, diff, change)
+        surf = [("fricVel", "FRICVEL"), ("htFlux", "HTFLUX"), ("tskin", "TSKIN")]
+        if moist:
+            surf += [("qFlux", "QFLUX"), ("qskin", "QSKIN")]
+        # Surface fields difference near-equal quantities (skin minus air temperature): the bound is
+        # twice the FP32 oracle's own deviation from FP64 plus 1e-6 of the field's range.
+        o32.steps(3)
+        for hname, oname in surf:
+            g = m.array(hname)[NH:-NH, NH:-NH].astype(np.float64)
+            o = o64.arr(oname)[NH:-NH, NH:-NH]
+            dev32 = np.abs(o32.arr(oname)[NH:-NH, NH:-NH] - o).max()
+            diff = np.abs(g - o).max()
+            print(f"{name} {hname}: |gpu-fp64| {diff:.3e}, |fp32-fp64| {dev32:.3e}")
+            assert diff <= 2 * dev32 + 1e-6 * np.abs(o).max(), (hname, diff, dev32)
+        o32.close()
+        o64.close()
+    finally:
+        m.close()

@@ -9,9 +9,20 @@ function is pinned by mathematics:
   Archimedes buoyancy and the ideal-gas pressure-gradient force; Rayleigh
   sin^2 profile; neutral surface-layer drag coefficient and Businger-Dyer
   unstable functions; SGS length-scale branches, pure-shear stress, N^2 of a
-  linear theta and Moeng's TKE dissipation; Flatau saturation vapour pressure
-  values, the saturation threshold and the water/heat budgets of saturation
-  adjustment; FP32 vs FP64 agreement.
+  linear theta and Moeng's TKE dissipation; TKE shear production of a pure
+  shear and the TKE turbulent transport of e = (a + b z)^2 in closed form; the
+  6th-order w filter on the 2-dz mode, on quartics and (worked by hand) its
+  limiter at a step; Flatau saturation vapour pressure values, the saturation
+  threshold and the water/heat budgets of saturation adjustment, the
+  vapour-density excess of condensation form 1 and the Soong-Ogura
+  linearised adjustment of form 2 (it lands within 10 % of saturation, form 1
+  overshoots); the neutral moist surface layer (C_q, moisture flux, lowest-face
+  moisture flux); FP32 vs FP64 agreement.
+
+Each pin was checked to fail on a planted mistake in the oracle (a factor 2 in
+the transport divergence, a wrong eddy-viscosity index, a flipped production
+sign, the sign of zero in the limiter, a dropped 1/rho or T^2 in form 2, z0m for
+z0t in C_q, the 273.16 offset of the e_s polynomial).
 """
 import math
 
@@ -536,4 +547,204 @@ def test_saturation_adjustment_threshold_and_budgets(tmp_path):
     np.testing.assert_allclose(Fm[1], 1e3 * fc, rtol=1e-12)
     Fth = I(o.arr("HYDRO_F"))[4]
     np.testing.assert_allclose(Fth, I(rth / rho / T) * Lv / cp * fc, rtol=1e-9, atol=1e-15)
+    o.close()
+
+
+def _filter_delta(wcol, coeff=0.25, dt=0.05):
+    """F_w with the 6th-order filter on minus F_w with it off, for rho*w = wcol(z) in every column
+    of a quiescent, uniform, neutral atmosphere (the filter is the only difference)."""
+    p = small(Nx=6, Ny=6, Nz=len(wcol) - 2 * NH, surflayerSelector=0, coriolisSelector=0, dampingLayerSelector=0,
+              turbulenceSelector=0, TKESelector=0, filter_6th_coeff=coeff, dt=dt, d_zeta=10.0)
+    out = []
+    for sel in (1, 0):
+        o = _uniform_oracle(dict(p, filterSelector=sel))
+        o.arr("HYDRO")[3] = np.asarray(wcol, dtype=np.float64)[None, None, :]
+        o.build_frhs(0)
+        out.append(o.arr("HYDRO_F")[3][NH + 1, NH + 1].copy())
+        o.close()
+    return out[0] - out[1], float(np.float32(coeff)) / float(np.float32(dt))
+
+
+def test_filter_6th_damps_the_2dz_mode_and_passes_quartics():
+    """The filter flux difference is the 6th difference (1,-6,15,-20,15,-6,1) times coeff/(64 dt):
+    the 2-dz mode A(-1)^k is removed at the rate coeff/dt (F = -(coeff/dt) rho w), and any
+    quartic profile in z is left untouched (the 5th differences that form the fluxes vanish)."""
+    nz = 24 + 2 * NH
+    k = np.arange(nz)
+    d, rate = _filter_delta(0.3 * (-1.0) ** k)
+    inner = slice(NH + 3, nz - NH - 3)
+    np.testing.assert_allclose(d[inner], -rate * 0.3 * (-1.0) ** k[inner], rtol=1e-12)
+    zz = (k - 10.3) / 7.0
+    d, _ = _filter_delta(0.2 + 0.1 * zz - 0.05 * zz ** 2 + 0.02 * zz ** 3 - 0.01 * zz ** 4)
+    assert np.abs(d[inner]).max() < 1e-12
+
+
+def test_filter_6th_limiter_at_a_step():
+    """rho*w = 0 below level s and 1 from s up.  Face fluxes (10, -5, 1 weights on the 5th
+    difference) at the faces s-3/2 .. s+5/2 are 1, -4, 6, -4, 1; a flux passes only where the IEEE
+    product flux*(face difference) is +0 or positive (the reference's fmaxf(copysign(1, g f), 0)),
+    so at the zero-difference faces the +1 fluxes pass and the -4 ones are blocked: limited fluxes
+    1, 0, 6, 0, 1, worked by hand.  Cell tendencies s-3 .. s+2 are then (1, -1, 6, -6, 1, -1) x
+    coeff/(64 dt) where the unlimited 6th difference gives (1, -5, 10, -10, 5, -1); zero beyond, and
+    zero in sum over the step (the filter conserves rho*w; the lid halo makes a second step)."""
+    nz = 20 + 2 * NH
+    ks = NH + 10
+    w = np.where(np.arange(nz) >= ks, 1.0, 0.0)
+    d, rate = _filter_delta(w)
+    expect = np.array([0, 1, -1, 6, -6, 1, -1, 0]) * rate / 64.0
+    np.testing.assert_allclose(d[ks - 4:ks + 4], expect, rtol=1e-12, atol=1e-15)
+    assert abs(d[ks - 4:ks + 4].sum()) < 1e-12
+
+
+def _tke_column(e_of_z, u_of_z=None, dz=10.0, nz=16):
+    """FP64 oracle, quiescent (or horizontally sheared) neutral column with unit density on a
+    10 m cube grid: sets rho e = e(z) and rho u = u(z), returns (F_e along z, z, Delta)."""
+    p = small(Nx=6, Ny=6, Nz=nz, surflayerSelector=0, coriolisSelector=0, dampingLayerSelector=0,
+              d_xi=dz, d_eta=dz, d_zeta=dz, c_k=0.10)
+    o = _uniform_oracle(p)
+    z = o.arr("ZPOS")[0][NH + 1, NH + 1].copy()
+    o.arr("TKE")[0] = e_of_z(z)[None, None, :]
+    if u_of_z is not None:
+        o.arr("HYDRO")[1] = u_of_z(z)[None, None, :]
+    o.build_frhs(2)
+    Fe = o.arr("TKE_F")[0][NH + 1, NH + 1].copy()
+    o.close()
+    return Fe, z, dz
+
+
+def test_tke_shear_production():
+    """Pure shear u = alpha z, uniform e, neutral: l = Delta, K_m = c_k Delta sqrt(e), and the TKE
+    budget is shear production K_m alpha^2 (= -tau_13 du/dz with tau_13 = -K_m alpha) minus
+    Moeng's dissipation 0.93 e^1.5 / Delta; no buoyancy production, no transport."""
+    e0, alpha = 0.3, 0.02
+    Fe, z, delta = _tke_column(lambda z: e0 + 0 * z, lambda z: alpha * z)
+    km = float(np.float32(0.10)) * delta * math.sqrt(e0)
+    expect = km * alpha ** 2 - 0.93 * e0 ** 1.5 / delta
+    inner = slice(NH + 2, len(z) - NH - 2)
+    np.testing.assert_allclose(Fe[inner], expect, rtol=1e-10)
+
+
+def test_tke_turbulent_transport_exact_for_linear_sqrt_e():
+    """e = (a + b z)^2 in a quiescent neutral column: grad e = 2b(a+bz) is exact, K_m sits on
+    the lower cell face (mean of l sqrt(e) over the two cells), K_m(z - dz/2) = c_k Delta (a + b(z - dz/2)),
+    and the transport is the divergence of the flux 2 K_m grad e (the reference's
+    (F(k+1) - F(k-1))/dz form): 2 d/dz [2b(a+bz) c_k Delta (a + b(z - dz/2))]
+    = 4 b^2 c_k Delta (2a + 2bz - b dz/2), on top of the dissipation 0.93 e^1.5/Delta."""
+    a, b = 0.4, 0.004
+    Fe, z, dz = _tke_column(lambda z: (a + b * z) ** 2)
+    delta = dz
+    ck = float(np.float32(0.10))
+    expect = 4 * b * b * ck * delta * (2 * a + 2 * b * z - b * dz / 2) - 0.93 * (a + b * z) ** 3 / delta
+    inner = slice(NH + 2, len(z) - NH - 2)
+    np.testing.assert_allclose(Fe[inner], expect[inner], rtol=1e-10)
+
+
+def _moist_column_oracle(tmp_path, **kw):
+    """FP64 oracle on the moist base state of an 8x8x10 case at rest (host set-up for the base
+    state), with saturation quantities of every cell from the ideal-gas law and the tabulated e_s."""
+    p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=10, thetaPerturbationSwitch=0, surflayerSelector=0, coriolisSelector=0,
+             dampingLayerSelector=0)
+    p.update(kw)
+    m = host_model(p, tmp_path)
+    o = oracle_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    f32 = lambda v: float(np.float32(v))
+    R, Rv, cv, Lv, p0 = f32(287.04), f32(461.6), f32(718.0), f32(2.5e6), 1.0e5
+    cp = R + cv
+    h = o.arr("HYDRO")
+    h[1:4] = 0.0
+    h[0] = o.arr("BS")[0]
+    h[4] = o.arr("BS")[1]
+    o.arr("TKE")[0] = 0.0
+    rho, rth = h[0], h[4]
+    pres = (R * rth / p0 ** (R / cp)) ** (cp / cv)
+    T = (rth / rho) * (pres / p0) ** (R / cp)
+    es = _es_of_T(T)
+    sat = dict(rho=rho, T=T, es=es, rhovs=es / (Rv * T), Lv=Lv, cp=cp, Rv=Rv)
+    return o, sat
+
+
+def _es_of_T(T):
+    """Flatau et al. (1992) liquid-water e_s [Pa]; the reference evaluates the polynomial at
+    T - 273.16 (cuda_moistureDevice.cu:159), which moves e_s by 0.07 %."""
+    a = [6.11239921, 0.443987641, 0.142986287e-1, 0.264847430e-3, 0.302950461e-5, 0.206739458e-7,
+         0.640689451e-10, -0.952447341e-13, -0.976195544e-15]
+    return sum(c * (T - 273.16) ** n for n, c in enumerate(a)) * 100.0
+
+
+def test_condensation_form_1_vapour_density_excess(tmp_path):
+    """moistureCond = 1 (full-pressure form, moistureCondBasePres = 0): the condensation rate is
+    the vapour-density excess over the saturation vapour density e_s/(R_v T) of the ideal gas; a
+    subsaturated cell evaporates its liquid up to the deficit (whichever is smaller), and a dry
+    subsaturated cell does nothing."""
+    o, s = _moist_column_oracle(tmp_path, moistureCond=1, moistureCondBasePres=0)
+    rvs = s["rhovs"]
+    k = np.arange(o.arr("HYDRO").shape[1])[:, None, None]
+    case = k % 4                            # 0: +1% over; 1: 2% under, no liquid; 2: 2% under, ample liquid; 3: 2% under, little liquid
+    fac = np.where(case == 0, 1.01, 0.98)
+    liq = np.select([case == 2, case == 3], [0.05 * rvs, 0.005 * rvs], 0.0)
+    mq = o.arr("MOIST")
+    mq[0] = 1e3 * rvs * fac
+    mq[1] = 1e3 * liq
+    o.build_frhs(2)
+    fc = I(o.arr("FCOND")).reshape(I(rvs).shape)
+    c, r = I(np.broadcast_to(case, rvs.shape)), I(rvs)
+    np.testing.assert_allclose(fc[c == 0], 0.01 * r[c == 0], rtol=1e-6)
+    assert np.all(fc[c == 1] == 0.0)
+    np.testing.assert_allclose(fc[c == 2], -0.02 * r[c == 2], rtol=1e-6)
+    np.testing.assert_allclose(fc[c == 3], -0.005 * r[c == 3], rtol=1e-9)
+    o.close()
+
+
+def test_condensation_form_2_reaches_saturation(tmp_path):
+    """moistureCond = 2: the linearised saturation adjustment (Soong and Ogura 1973)
+    delta = (rho_v - rho_vs) / (1 + L_v^2 q_vs / (c_p R_v T^2)), q_vs = rho_vs/rho.  Physically: after
+    condensing delta at constant density and warming by L_v delta/(rho c_p), the parcel is within a
+    tenth of its initial excess of saturation (the linearisation drops the -rho_vs/T part of
+    d rho_vs/dT), whereas form 1 (all of the excess at once) overshoots to subsaturation."""
+    o, s = _moist_column_oracle(tmp_path, moistureCond=2, moistureCondBasePres=0)
+    rvs, rho, T = s["rhovs"], s["rho"], s["T"]
+    excess = 0.02 * rvs
+    o.arr("MOIST")[0] = 1e3 * (rvs + excess)
+    o.arr("MOIST")[1] = 0.0
+    o.build_frhs(2)
+    fc = o.arr("FCOND").reshape(rvs.shape)
+    qvs = rvs / rho
+    expect = excess / (1 + s["Lv"] ** 2 * qvs / (s["cp"] * s["Rv"] * T ** 2))
+    np.testing.assert_allclose(I(fc), I(expect), rtol=1e-6)
+
+    def residual(delta):
+        T2 = T + s["Lv"] * delta / (rho * s["cp"])
+        return (rvs + excess - delta) - _es_of_T(T2) / (s["Rv"] * T2)
+
+    r2 = I(residual(fc) / excess)
+    r1 = I(residual(excess) / excess)
+    assert np.all(np.abs(r2) < 0.1), np.abs(r2).max()
+    assert np.all(r1 < -0.5), r1.max()
+    o.close()
+
+
+def test_surface_layer_moist_neutral_transfer(tmp_path):
+    """Moist surface layer with a skin temperature and humidity (surflayerSelector = 2), neutral
+    (surflayer_stab = 1): C_q = C_h = kappa^2 / ln^2((z1+z0t)/z0t), the moisture flux is
+    C_q |U1| (q_skin - q_1) and the lowest-face SGS moisture flux is rho times it."""
+    o, s = _moist_column_oracle(tmp_path, surflayerSelector=2, surflayer_stab=1, surflayer_tr=0.0,
+                                surflayer_qr=0.0, U_g=-6.0, surflayer_z0t=0.01)
+    h = o.arr("HYDRO")
+    h[1] = -6.0 * h[0]
+    h[2] = 2.0 * h[0]
+    o.arr("MOIST")[0] = 10.0 * h[0]
+    o.arr("MOIST")[1] = 0.0
+    o.arr("QSKIN")[...] = 14.0
+    o.build_frhs(2)
+    f32 = lambda v: float(np.float32(v))
+    z1, z0t, kap = 0.5 * 25.0, f32(0.01), f32(0.4)
+    cq = kap ** 2 / math.log((z1 + z0t) / z0t) ** 2
+    np.testing.assert_allclose(o.arr("CQ")[NH:-NH, NH:-NH], cq, rtol=1e-12)
+    np.testing.assert_allclose(o.arr("CH")[NH:-NH, NH:-NH], cq, rtol=1e-12)
+    U1 = math.hypot(6.0, 2.0)
+    np.testing.assert_allclose(o.arr("QFLUX")[NH:-NH, NH:-NH], cq * U1 * (14.0 - 10.0), rtol=1e-12)
+    rho1 = h[0][NH:-NH, NH:-NH, NH]
+    np.testing.assert_allclose(o.arr("MOISTTAU")[2][NH:-NH, NH:-NH, NH], cq * U1 * 4.0 * rho1, rtol=1e-12)
     o.close()

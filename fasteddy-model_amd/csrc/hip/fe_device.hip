@@ -465,6 +465,13 @@ int cuda_hydroCoreInit(void) {
   D.nTKE = (turbulenceSelector > 0 && TKESelector > 0) ? 1 : 0;
   D.nM = moistureSelector > 0 ? moistureNvars : 0;
   D.NF = 5 + D.nTKE + D.nM;
+  /* the stage kernels address all fields of a state array through one buffer
+   * resource with 32-bit offsets */
+  if ((long)D.NF * D.fs * (long)sizeof(float) > 0xFFFFFFFFL) {
+    fprintf(stderr, "cuda_hydroCoreInit: %d fields of %d x %d x %d exceed 4 GiB per state array\n", D.NF, D.Nx, D.Ny,
+            D.Nz);
+    return FE_ERR_UNSUPPORTED;
+  }
   const long fsH = D.n3h;
   const float *rbs = hydroBaseStateFlds, *tbs = hydroBaseStateFlds + fsH;
   const int i0 = iMin, j0 = jMin;

@@ -33,6 +33,8 @@ constexpr int RW = TX + 2 * HH, RH = TY + 2 * HH;      /* raw plane with 3-cell 
 constexpr int DW = TX + 2, DH = TY + 2;                /* ring-1 region [-1,TX] x [-1,TY]     */
 constexpr int NHALO = RW * RH - TX * TY;               /* halo cells of the raw plane (<= NT) */
 constexpr int NR1 = DW * DH - TX * TY;                 /* ring-1 cells, listed first          */
+/* third word of a raw (stride 0) buffer resource on CDNA: 32-bit data format, no swizzle */
+constexpr int FE_RSRC_W3 = 0x00020000;
 static_assert(NHALO <= NT, "one halo cell per thread");
 
 __device__ __forceinline__ int wrapi(int v, int n) {
@@ -387,8 +389,8 @@ __device__ __forceinline__ void dyn_tendency(const StepArgs &a, int z, float (*w
 
 /* RK3-WS2002 stage update of the own cell (cuda_RKschemes.cu:113-157, positivity of
  * rho*e and moisture), or the tendency itself in emit mode. */
-template <class C, bool EMIT>
-__device__ __forceinline__ void rk_store(const StepArgs &a, char *const *fout, unsigned oz, const float *F,
+template <class C, bool EMIT, class TL>
+__device__ __forceinline__ void rk_store(const StepArgs &a, const TL &T, unsigned oz, const float *F,
                                          const float *base) {
   if (EMIT && a.fem) {
 #pragma unroll
@@ -396,14 +398,14 @@ __device__ __forceinline__ void rk_store(const StepArgs &a, char *const *fout, u
   }
   if (a.emitF) {
 #pragma unroll
-    for (int f = 0; f < C::NF; ++f) *(float *)(fout[f] + oz) = F[f];
+    for (int f = 0; f < C::NF; ++f) T.st_out(f, oz, F[f]);
   } else {
 #pragma unroll
     for (int f = 0; f < C::NF; ++f) {
       float v = fmaf(a.coefdt, F[f], base[f]);
       if (C::TKE && f == C::IE) v = fmaxf(v, 1e-10f);
       if (C::NM > 0 && f >= C::IQV) v = fmaxf(v, 0.f);
-      *(float *)(fout[f] + oz) = v;
+      T.st_out(f, oz, v);
     }
   }
 }
@@ -448,8 +450,8 @@ __device__ __forceinline__ void sgs_lq(const StepArgs &a, int z, float thm, floa
 }
 
 /* Per-thread set-up shared by the stage kernels: tile position (XCD-aware
- * order), own column, this thread's halo cell, per-field base pointers and the
- * upwind weights. */
+ * order), own column, this thread's halo cell, buffer resources of the state
+ * arrays and the upwind weights. */
 template <class C>
 struct Tile {
   static constexpr int NF = C::NF;
@@ -457,11 +459,13 @@ struct Tile {
   bool active, baseIn, hasH, hasR1;
   int hlx, hly, hcol, hslot, hd, od;
   unsigned pb, colb, hcolb;
-  /* Global accesses use a wave-uniform per-field base pointer plus one 32-bit
-   * byte offset per thread (a field is < 4 GiB), so one offset register serves
-   * every field and the loads take the scalar-base addressing form. */
-  const char *fin[NF], *fbs[NF];
-  char *fout[NF];
+  /* Global accesses go through one buffer resource per array (stage input, RK
+   * base, stage output) with the field's byte offset f*fsb in the instruction's
+   * scalar offset and one 32-bit column offset per thread: three descriptors and
+   * one stride in scalar registers instead of a 64-bit pointer per field and
+   * array (all fields of an array span < 4 GiB, checked at set-up). */
+  __amdgpu_buffer_rsrc_t rin, rbs, rout;
+  unsigned fsb;
   float omb[NF];               /* (1 - b) of the hybrid schemes, 0 for the others */
 
   __device__ __forceinline__ explicit Tile(const StepArgs &a) {
@@ -480,12 +484,10 @@ struct Tile {
     cy = wrapi(y0 + ty, a.Ny);
     col = cy * a.P + cx;
     baseIn = (a.base == a.in);             /* first RK stage: phi^n is the stage input */
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      fin[f] = (const char *)(a.in + (long)f * a.fs);
-      fbs[f] = (const char *)(a.base + (long)f * a.fs);
-      fout[f] = (char *)(a.out + (long)f * a.fs);
-    }
+    rin = __builtin_amdgcn_make_buffer_rsrc((void *)a.in, 0, -1, FE_RSRC_W3);
+    rbs = __builtin_amdgcn_make_buffer_rsrc((void *)a.base, 0, -1, FE_RSRC_W3);
+    rout = __builtin_amdgcn_make_buffer_rsrc((void *)a.out, 0, -1, FE_RSRC_W3);
+    fsb = (unsigned)a.fs * 4u;
     pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
     /* this thread's halo cell */
     hlx = 0; hly = 0; hcol = 0; hslot = -1;
@@ -508,13 +510,23 @@ struct Tile {
     }
   }
 
+  __device__ __forceinline__ float ld_in(int f, unsigned o) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rin, o, (unsigned)f * fsb, 0));
+  }
+  __device__ __forceinline__ float ld_bs(int f, unsigned o) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rbs, o, (unsigned)f * fsb, 0));
+  }
+  __device__ __forceinline__ void st_out(int f, unsigned o, float v) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rout, o, (unsigned)f * fsb, 0);
+  }
+
   /* every field of level zz of the column at byte offset cb, with the vertical
    * boundary rules applied for levels outside [0, Nz) (one uniform branch) */
   __device__ __forceinline__ void lvl(const StepArgs &a, float (&v)[NF], int zz, unsigned cb) const {
     if (zz >= 0 && zz < a.Nz) {
       const unsigned o = (unsigned)zz * pb + cb;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) v[f] = *(const float *)(fin[f] + o);
+      for (int f = 0; f < NF; ++f) v[f] = ld_in(f, o);
     } else {
       const bool top = zz >= a.Nz;
       const unsigned o = top ? (unsigned)(a.Nz - 1) * pb + cb : cb;
@@ -525,7 +537,7 @@ struct Tile {
         else if (top && f == 1) v[f] = a.uTop;
         else if (top && f == 2) v[f] = a.vTop;
         else if (top && (f == 3 || (C::TKE && f == C::IE))) v[f] = 0.f;
-        else v[f] = *(const float *)(fin[f] + o);
+        else v[f] = ld_in(f, o);
       }
     }
   }
@@ -588,10 +600,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   const bool active = T.active, baseIn = T.baseIn, hasH = T.hasH, hasR1 = T.hasR1;
   const int hslot = T.hslot, hd = T.hd, od = T.od;
   const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
-  const char *const *fbs = T.fbs;
-  char *const *fout = T.fout;
   const float *omb = T.omb;
-  auto ld = [](const char *bp, unsigned off) -> float { return *(const float *)(bp + off); };
   auto lvl = [&](float (&v)[NF], int zz, unsigned cb) { T.lvl(a, v, zz, cb); };
   auto mfac = [&](float rho, float qv) { return T.mfac(a, rho, qv); };
 
@@ -708,7 +717,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   for (int f = 0; f < NF; ++f) bse[f] = bsn[f] = 0.f;
   if (!a.emitF && !baseIn) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) bse[f] = ld(fbs[f], colb);
+    for (int f = 0; f < NF; ++f) bse[f] = T.ld_bs(f, colb);
   }
 
   for (int z = 0; z < Nz; ++z) {
@@ -726,7 +735,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     if (hasH) lvl(hv[PD], z + PD, hcolb);
     if (z + 1 < Nz && !a.emitF && !baseIn) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bsn[f] = ld(fbs[f], oz + pb);
+      for (int f = 0; f < NF; ++f) bsn[f] = T.ld_bs(f, oz + pb);
     }
 
     /* ---- P1 ---- */
@@ -1035,7 +1044,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float b0[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) b0[f] = baseIn ? win[f][3] : bse[f];
-        rk_store<C, EMIT>(a, fout, oz, F, b0);
+        rk_store<C, EMIT>(a, T, oz, F, b0);
         if (a.pout) *(float *)((char *)a.pout + oz) = pz[1];
       }
 #pragma unroll
@@ -1165,7 +1174,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
     if (z2 >= 0 && z2 < Nz && !a.emitF && !T.baseIn) {
       const unsigned o2 = (unsigned)z2 * pb + colb;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) bs[S + 1][f] = *(const float *)(T.fbs[f] + o2);
+      for (int f = 0; f < NF; ++f) bs[S + 1][f] = T.ld_bs(f, o2);
     }
     {
       float t[NF];
@@ -1236,7 +1245,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
         float b0[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3 + S] : bs[S][f];
-        rk_store<C, EMIT>(a, T.fout, o3, F, b0);
+        rk_store<C, EMIT>(a, T, o3, F, b0);
         if (a.pout) *(float *)((char *)a.pout + o3) = pp[S + 1];
       }
     } else {

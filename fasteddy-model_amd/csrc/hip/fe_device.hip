@@ -758,6 +758,140 @@ int fe_profiles(double *out, int nstat) {
   return FE_OK;
 }
 
+/* Per-field state summary of the device state, the on-device counterpart of the
+ * reference's host scan hydro_coreStateLogDump / hydro_coreFldStateLogEntry
+ * (SRC/HYDRO_CORE/hydro_core.c:1282-1525): over the interior cells, the maximum
+ * and minimum with their first location in the reference's (i, j, k) loop order,
+ * the mean over all cells of the finite values, and the NaN and Inf counts, of
+ * rho, u, v, w, theta (rho-weighted fields times float(1/rho), as the reference
+ * forms them), p', the SGS TKE and the moisture species per unit mass; plus the
+ * horizontal sum of the raw (rho-weighted) field, from which the host forms its
+ * volume integral (mass, rho*theta, ... budgets).  One work-group per level; the
+ * FP64 sums and the (value, location) extremes are reduced with cross-lane
+ * shuffles, then across the wavefronts in LDS; the host combines the levels. */
+constexpr int SUM_NT = 256, SUM_NCOL = 12, SUM_LV = 8, SUM_MAXF = 9;
+__device__ __forceinline__ void sum_pick(float &v, long long &k, float v2, long long k2, bool is_max) {
+  const bool take = is_max ? (v2 > v || (v2 == v && k2 < k)) : (v2 < v || (v2 == v && k2 < k));
+  if (take) { v = v2; k = k2; }
+}
+__global__ void __launch_bounds__(SUM_NT) k_summary(const float *S, const float *pres, long fs, long ps, int Nx, int Ny, int Nz,
+                                                     int P, int ns, int nTKE, double *out) {
+  const int z = blockIdx.x;
+  float mx[SUM_MAXF], mn[SUM_MAXF];
+  long long kx[SUM_MAXF], kn[SUM_MAXF];
+  double sm[SUM_MAXF], raw[SUM_MAXF];
+  int nnan[SUM_MAXF], ninf[SUM_MAXF];
+  const long long NONE = 0x7fffffffffffffffLL;
+#pragma unroll
+  for (int f = 0; f < SUM_MAXF; ++f) {
+    mx[f] = -1.0e16f; mn[f] = 1.0e16f; kx[f] = kn[f] = NONE; sm[f] = raw[f] = 0.0; nnan[f] = ninf[f] = 0;
+  }
+  const int n = Nx * Ny;
+  for (int c = threadIdx.x; c < n; c += SUM_NT) {
+    const int y = c / Nx, x = c - y * Nx;
+    const long o = (long)z * ps + (long)y * P + x;
+    const long long key = ((long long)x * Ny + y) * Nz + z;
+    const float rho = S[o];
+    const float rinv = (float)(1.0 / (double)rho);
+#pragma unroll
+    for (int f = 0; f < SUM_MAXF; ++f) {
+      if (f >= ns) break;
+      /* rows: rho, u, v, w, theta, p', TKE..., moisture... */
+      float fr;
+      if (f < 5) fr = S[(long)f * fs + o];
+      else if (f == 5) fr = pres ? pres[o] : 0.f;
+      else fr = S[(long)(f - 1) * fs + o];
+      const bool cons = (f != 0 && f != 5);
+      const float v = cons ? fr * rinv : fr;
+      raw[f] += (double)fr;
+      if (isnan(v)) { nnan[f]++; continue; }
+      if (isinf(v)) { ninf[f]++; continue; }
+      sum_pick(mx[f], kx[f], v, key, true);
+      sum_pick(mn[f], kn[f], v, key, false);
+      sm[f] += (double)v;
+    }
+  }
+  /* wavefront reduction */
+#pragma unroll
+  for (int f = 0; f < SUM_MAXF; ++f) {
+    for (int off = 32; off > 0; off >>= 1) {
+      sum_pick(mx[f], kx[f], __shfl_xor(mx[f], off), __shfl_xor(kx[f], off), true);
+      sum_pick(mn[f], kn[f], __shfl_xor(mn[f], off), __shfl_xor(kn[f], off), false);
+      sm[f] += __shfl_xor(sm[f], off);
+      raw[f] += __shfl_xor(raw[f], off);
+      nnan[f] += __shfl_xor(nnan[f], off);
+      ninf[f] += __shfl_xor(ninf[f], off);
+    }
+  }
+  __shared__ double part[SUM_NT / 64][SUM_MAXF][SUM_LV];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) {
+#pragma unroll
+    for (int f = 0; f < SUM_MAXF; ++f) {
+      double *q = part[wave][f];
+      q[0] = mx[f]; q[1] = (double)kx[f]; q[2] = mn[f]; q[3] = (double)kn[f];
+      q[4] = sm[f]; q[5] = nnan[f]; q[6] = ninf[f]; q[7] = raw[f];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < ns) {
+    const int f = threadIdx.x;
+    float vx = (float)part[0][f][0], vn = (float)part[0][f][2];
+    long long qx = (long long)part[0][f][1], qn = (long long)part[0][f][3];
+    double a4 = part[0][f][4], a5 = part[0][f][5], a6 = part[0][f][6], a7 = part[0][f][7];
+    for (int wv = 1; wv < SUM_NT / 64; ++wv) {
+      sum_pick(vx, qx, (float)part[wv][f][0], (long long)part[wv][f][1], true);
+      sum_pick(vn, qn, (float)part[wv][f][2], (long long)part[wv][f][3], false);
+      a4 += part[wv][f][4]; a5 += part[wv][f][5]; a6 += part[wv][f][6]; a7 += part[wv][f][7];
+    }
+    double *r = out + ((long)z * ns + f) * SUM_LV;
+    r[0] = vx; r[1] = (double)qx; r[2] = vn; r[3] = (double)qn; r[4] = a4; r[5] = a5; r[6] = a6; r[7] = a7;
+  }
+}
+
+int fe_state_summary(double *out, int nrow) {
+  if (!D.hydro) return FE_ERR_STATE;
+  const int ns = 6 + D.nTKE + D.nM;
+  if (nrow != ns || !out || ns > SUM_MAXF) return FE_ERR_UNSUPPORTED;
+  std::vector<double> lv((size_t)D.Nz * ns * SUM_LV);
+  double *d = nullptr;
+  HIPCHK(hipMalloc(&d, lv.size() * sizeof(double)));
+  hipLaunchKernelGGL(k_summary, dim3(D.Nz), dim3(SUM_NT), 0, D.stream, D.S[0], D.pout, D.fs, D.ps, D.Nx, D.Ny, D.Nz, D.P, ns,
+                     D.nTKE, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(lv.data(), d, lv.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
+  hipFree(d);
+  HIPCHK(e);
+  const double NONE = (double)0x7fffffffffffffffLL;
+  const double ncell = (double)D.Nx * D.Ny * D.Nz;
+  const double dV = (double)dX * dY * dZ;
+  for (int f = 0; f < ns; ++f) {
+    float vx = -1.0e16f, vn = 1.0e16f;
+    long long qx = 0x7fffffffffffffffLL, qn = qx;
+    double sum = 0, nn = 0, ni = 0, integ = 0;
+    for (int z = 0; z < D.Nz; ++z) {
+      const double *r = &lv[((size_t)z * ns + f) * SUM_LV];
+      const float v1 = (float)r[0], v2 = (float)r[2];
+      const long long k1 = (long long)r[1], k2 = (long long)r[3];
+      if (v1 > vx || (v1 == vx && k1 < qx)) { vx = v1; qx = k1; }
+      if (v2 < vn || (v2 == vn && k2 < qn)) { vn = v2; qn = k2; }
+      sum += r[4]; nn += r[5]; ni += r[6];
+      integ += r[7] * (double)D_Jac[idx3(Nh, Nh, z + Nh)] * dV;
+    }
+    double *o = out + (long)f * SUM_NCOL;
+    auto loc = [&](long long q, double *ijk) {
+      if ((double)q >= NONE) { ijk[0] = ijk[1] = ijk[2] = 0; return; }
+      const long long zk = q % D.Nz, rest = q / D.Nz, yj = rest % D.Ny, xi = rest / D.Ny;
+      ijk[0] = (double)(xi + Nh); ijk[1] = (double)(yj + Nh); ijk[2] = (double)(zk + Nh);
+    };
+    o[0] = vx; loc(qx, o + 1);
+    o[4] = vn; loc(qn, o + 5);
+    o[8] = sum / ncell; o[9] = nn; o[10] = ni; o[11] = integ;
+  }
+  return FE_OK;
+}
+
 int fe_debug_build_frhs(int stage) {
   if (!D.hydro) return FE_ERR_STATE;
   int rc;

@@ -163,19 +163,27 @@ struct Cfg {
   }
 };
 
+#ifndef FE_LDSMODE
+#define FE_LDSMODE 0
+#endif
+#if FE_LDSMODE == 1
+typedef volatile __attribute__((address_space(3))) float SRD;
+#else
+typedef __attribute__((address_space(3))) float SRD;
+#endif
 template <class C, int F, int FE = C::NF>
 struct FaceLoop {
   __device__ __forceinline__ static void xf(float (*raw)[RH][RW], float (*gx)[TY][TX + 1], int fy, int fx, float U, const float *omb) {
     if constexpr (F < FE) {
-      const float *r = &raw[F][fy + HH][fx];
+      const SRD *r = (const SRD *)&raw[F][fy + HH][fx];
       gx[F][fy][fx] = face_flux<C::template sch<F>()>(U, omb[F], r[0], r[1], r[2], r[3], r[4], r[5]);
       FaceLoop<C, F + 1, FE>::xf(raw, gx, fy, fx, U, omb);
     }
   }
   __device__ __forceinline__ static void yf(float (*raw)[RH][RW], float (*gy)[TY + 1][TX], int fy, int fx, float V, const float *omb) {
     if constexpr (F < FE) {
-      gy[F][fy][fx] = face_flux<C::template sch<F>()>(V, omb[F], raw[F][fy][fx + HH], raw[F][fy + 1][fx + HH], raw[F][fy + 2][fx + HH],
-                                                      raw[F][fy + 3][fx + HH], raw[F][fy + 4][fx + HH], raw[F][fy + 5][fx + HH]);
+      const SRD *r = (const SRD *)&raw[F][fy][fx + HH];
+      gy[F][fy][fx] = face_flux<C::template sch<F>()>(V, omb[F], r[0], r[RW], r[2 * RW], r[3 * RW], r[4 * RW], r[5 * RW]);
       FaceLoop<C, F + 1, FE>::yf(raw, gy, fy, fx, V, omb);
     }
   }
@@ -1167,7 +1175,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
   auto step = [&](auto slot, int it) {
     constexpr int S = decltype(slot)::value;
     const int z1 = it, z2 = it - 1, z3 = it - 2;
-    const int b1 = it & 1, b2 = b1 ^ 1;              /* copies of levels it (and it-2) and it-1 */
+    constexpr int b1 = S, b2 = S ^ 1;                /* copies of levels it (and it-2) and it-1; it has parity S */
     const unsigned o3 = (unsigned)z3 * pb + colb;
 
     /* ---- loads for the next level: RK base of level it-1, own level it+2, halo level it+2 ---- */
@@ -1237,8 +1245,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
 #pragma unroll
         for (int j = 0; j < 7; ++j) w7[f][j] = wv[f][j + S];
 #pragma unroll
-      for (int f = 0; f < NF; ++f)
-        dxy[f] = (gx[b1][f][ty][tx + 1] - gx[b1][f][ty][tx]) + (gy[b1][f][ty + 1][tx] - gy[b1][f][ty][tx]);
+      for (int f = 0; f < NF; ++f) {
+        const SRD *px = (const SRD *)&gx[b1][f][ty][tx], *py = (const SRD *)&gy[b1][f][ty][tx];
+        dxy[f] = (px[1] - px[0]) + (py[TX] - py[0]);
+      }
       dyn_tendency<C, 7>(a, z3, w7, ri[S], ri[S + 1], pp[S], pp[S + 2], pg[S][0], pg[S][1], dxy, gz[S], gz[S + 1],
                          T.omb, F);
       if (T.active) {

@@ -97,6 +97,7 @@ class Model:
             L.fe_debug_build_frhs.argtypes = [C.c_int]
             L.fe_device_steps.argtypes = [C.c_int]
             L.fe_profiles.argtypes = [C.POINTER(C.c_double), C.c_int]
+            L.fe_state_summary.argtypes = [C.POINTER(C.c_double), C.c_int]
             L.fe_rand.restype = C.c_int
             L.fe_srand.argtypes = [C.c_uint]
             L.getParameterErrors.restype = C.c_int
@@ -201,6 +202,26 @@ class Model:
         if rc != 0:
             raise RuntimeError(f"fe_profiles failed with {rc}")
         return out
+
+    SUMMARY_COLS = ("max", "imax", "jmax", "kmax", "min", "imin", "jmin", "kmin", "mean", "nan", "inf", "integral")
+
+    def summary_rows(self):
+        rows = ["rho", "u", "v", "w", "theta", "press"]
+        if self.geti("turbulenceSelector") > 0 and self.geti("TKESelector") > 0:
+            rows.append("TKE_0")
+        if self.geti("moistureSelector") > 0:
+            rows += ["qv", "ql", "qr"][: self.geti("moistureNvars")]
+        return rows
+
+    def state_summary(self):
+        """Per-field state summary of the device state (fe_state_summary): {row: {column: value}}
+        for the rows of summary_rows() and the columns SUMMARY_COLS."""
+        rows = self.summary_rows()
+        out = np.zeros((len(rows), len(self.SUMMARY_COLS)))
+        rc = self.L.fe_state_summary(out.ctypes.data_as(C.POINTER(C.c_double)), len(rows))
+        if rc != 0:
+            raise RuntimeError(f"fe_state_summary failed with {rc}")
+        return {r: dict(zip(self.SUMMARY_COLS, out[i])) for i, r in enumerate(rows)}
 
     def device_steps(self, n):
         rc = self.L.fe_device_steps(int(n))

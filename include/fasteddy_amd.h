@@ -84,6 +84,16 @@ const char *fe_kernel_info(void);
  * No counterpart in the reference's C code; it restates the reference's
  * post-processing (tutorials/notebooks/fetutorialfunctions.py:55-83,344-431). */
 int fe_profiles(double *out, int nstat);
+/* Per-field state summary of the current device state (the on-device counterpart
+ * of hydro_coreStateLogDump / hydro_coreFldStateLogEntry, SRC/HYDRO_CORE/hydro_core.c:1282-1525),
+ * out[nrow][12] with nrow = 6 + TKE fields + moisture species, rows rho, u, v, w,
+ * theta, p', TKE, q_v, q_l (rho-weighted fields per unit mass): max, its (i,j,k),
+ * min, its (i,j,k) (halo-inclusive indices, first occurrence in i-j-k loop order),
+ * mean over all interior cells of the finite values, NaN count, Inf count, and the
+ * volume integral of the raw field (sum of field * D_Jac * dX dY dZ: the mass,
+ * rho*theta, ... budgets).  FP64 sums on the device; returns FE_ERR_UNSUPPORTED
+ * for a wrong nrow. */
+int fe_state_summary(double *out, int nrow);
 
 #ifdef __cplusplus
 }

@@ -43,13 +43,14 @@ int fe_setup_host(const char *paramFile) {
     return getParameterErrors() > 0 ? getParameterErrors() : 1;
   }
   if (fempi_Init() || ioInit()) return -1;
+  host_ready = 1;                  /* from here on fe_teardown releases what was allocated */
   if (gridInit()) return -2;
   if (hydro_coreInit()) return -3;
   if (timeInit()) return -4;
+  if (inFile != NULL && ioReadBinaryinFileSingleTime()) return -8;    /* FastEddy.c:202-208 */
   if (gridSecondaryPreparations()) return -5;
   if (hydro_coreSetBaseState()) return -6;
-  if (hydro_corePrepareFromInitialConditions()) return -7;
-  host_ready = 1;
+  if (inFile != NULL && hydro_corePrepareFromInitialConditions()) return -7;
   return 0;
 }
 

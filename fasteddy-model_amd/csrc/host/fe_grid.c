@@ -71,13 +71,8 @@ int ioGetParams(void) {
   r = queryIntegerParameter("frqOutput", &frqOutput, 0, INT_MAX, PARAM_MANDATORY); if (r) rc = r;
   return rc;
 }
-int ioInit(void) {
-  if (inFile != NULL) {
-    printf("ERROR: restart/initial-condition input files (inFile=%s) need the netCDF reader, which this build does not include\n", inFile);
-    return -1;
-  }
-  return 0;
-}
+/* An inFile is this build's own binary output (fe_io.c), read back as restart input. */
+int ioInit(void) { return 0; }
 
 /* ---------------------------------------------------------------------- GRID */
 char *gridFile = NULL, *topoFile = NULL;

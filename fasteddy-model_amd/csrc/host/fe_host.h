@@ -72,6 +72,8 @@ extern char *inPath, *inFile, *outPath, *outFileBase;
 extern int frqOutput, ioOutputMode;
 int ioGetParams(void);
 int ioInit(void);
+int ioWriteBinaryoutFileSingleTime(int tstep, int Nx, int Ny, int Nz, int Nh);
+int ioReadBinaryinFileSingleTime(void);
 
 /* ---- GRID (SRC/GRID/grid.h) ----------------------------------------------- */
 extern char *gridFile, *topoFile;

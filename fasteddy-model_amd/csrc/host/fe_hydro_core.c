@@ -316,7 +316,9 @@ int hydro_coreSetBaseState(void) {
         thetaBase[n] = thetaBase[n] * rhoBase[n];
         hydroBaseStatePres[n] = pow(thetaBase[n] * constant_1, cp_cv);
       }
-  /* default initial conditions (no input file): geostrophic momentum, w = 0 */
+  /* default initial conditions, only without an input file (hydro_core.c:1192-1252):
+   * geostrophic momentum, w = 0, base-state rho and rho*theta, seeded perturbations */
+  if (inFile != NULL) return 0;
   float *uIni = &hydroFlds[U_INDX * fs], *vIni = &hydroFlds[V_INDX * fs], *wIni = &hydroFlds[W_INDX * fs];
   for (int i = iMin - Nh; i < iMax + Nh; i++)
     for (int j = jMin - Nh; j < jMax + Nh; j++)

@@ -7,6 +7,7 @@
 #include <limits.h>
 #include <float.h>
 #include <stdio.h>
+#include <string.h>
 #include "fe_host.h"
 
 int timeMethod, Nt, NtBatch, simTime_it, simTime_itRestart, numRKstages;
@@ -22,10 +23,22 @@ int timeGetParams(void) {
 }
 
 int timeInit(void) {
-  /* no input file: the run starts at step 0 (restart input needs the netCDF reader) */
-  simTime = 0.0f;
-  simTime_it = 0;
-  simTime_itRestart = simTime_it;
+  if (inFile == NULL) {
+    simTime = 0.0f;
+    simTime_it = 0;
+    simTime_itRestart = simTime_it;
+  } else {
+    /* restart step: the integer after the first '.' of the input file name
+     * (SRC/TIME_INTEGRATION/time_integration.c:99-105) */
+    const char *dot = strchr(inFile, '.');
+    simTime_itRestart = 0;
+    if (!dot || sscanf(dot + 1, "%d", &simTime_itRestart) != 1) {
+      printf("ERROR: no restart step after '.' in inFile = %s\n", inFile);
+      return -1;
+    }
+    simTime_it = simTime_itRestart;
+    simTime = dt * simTime_it;
+  }
   if (timeMethod == 0) numRKstages = 2;
   return 0;
 }

@@ -2,7 +2,9 @@
  * fe_main.c -- stand-alone single-rank driver: `fasteddy_amd case.in`.
  * Runs the batch loop of SRC/FEMAIN/FastEddy.c:333-396 (Nt steps in batches
  * of NtBatch) and prints the reference's per-batch timing table
- * (FastEddy.c:386-395) plus a per-field state summary.
+ * (FastEddy.c:386-395), with the per-field state summary and the binary output
+ * (ioOutputMode = 1) every frqOutput steps; an inFile written by this build
+ * restarts the run from its step.
  */
 #include <math.h>
 #include <stdio.h>
@@ -16,23 +18,48 @@ static double wall(void) {
   return t.tv_sec + 1e-9 * t.tv_nsec;
 }
 
-static void summary(void) {
-  const char *names[5] = {"rho", "u", "v", "w", "theta"};
-  const long n3 = (long)(Nxp + 2 * Nh) * (Nyp + 2 * Nh) * (Nzp + 2 * Nh);
-  for (int f = 0; f < 5; ++f) {
-    double mx = -1e30, mn = 1e30, sum = 0;
-    long cnt = 0, bad = 0;
-    for (int i = iMin; i < iMax; ++i)
-      for (int j = jMin; j < jMax; ++j)
-        for (int k = kMin; k < kMax; ++k) {
-          long n = ((long)i * (Nyp + 2 * Nh) + j) * (Nzp + 2 * Nh) + k;
-          double rho = hydroFlds[n];
-          double v = f == 0 ? rho : hydroFlds[f * n3 + n] / rho;
-          if (!isfinite(v)) { bad++; continue; }
-          mx = v > mx ? v : mx; mn = v < mn ? v : mn; sum += v; cnt++;
-        }
-    printf("  %-6s max %14.6e  min %14.6e  mean %14.6e  non-finite %ld\n", names[f], mx, mn, cnt ? sum / cnt : 0.0, bad);
+/* State summary in the reference's log format (hydro_coreStateLogDump,
+ * SRC/HYDRO_CORE/hydro_core.c:1282-1525), from the device reduction. */
+static void state_log(int it) {
+  const int nrow = 6 + ((turbulenceSelector > 0 && TKESelector > 0) ? 1 : 0) + (moistureSelector > 0 ? moistureNvars : 0);
+  double out[9 * 12];
+  if (nrow > 9 || fe_state_summary(out, nrow) != 0) {
+    printf("state summary unavailable\n");
+    return;
   }
+  const char *names[9] = {"rho", "u", "v", "w", "theta", "press"};
+  int nn = 6;
+  if (turbulenceSelector > 0 && TKESelector > 0) names[nn++] = "TKE_0";
+  if (moistureSelector > 0 && moistureNvars > 0) names[nn++] = "qv";
+  if (moistureSelector > 0 && moistureNvars > 1) names[nn++] = "ql";
+  printf("\n_____________________#######_________  STATE-SUMMARY @ it = %d _________#######____________________ \n", it);
+  printf("Field \t|\t max\t     --(i,j,k)\t\t |\t  min\t     --(i,j,k)\t\t |\t mean\n");
+  printf("---------------------------------------------------------------------------------------------------------\n");
+  for (int r = 0; r < nrow; ++r) {
+    const double *o = out + r * 12;
+    printf("%s\t", r < nn ? names[r] : "?");
+    if (o[9] == 0 && o[10] == 0)
+      printf("Rank %d/%d: %16.8f \t (%d,%d,%d)\t |  %16.8f \t (%d,%d,%d)\t | %16.8f \n", 0, 1, o[0], (int)o[1], (int)o[2],
+             (int)o[3], o[4], (int)o[5], (int)o[6], (int)o[7], o[8]);
+    else
+      printf("Rank %d/%d: ****CORRUPTED*** --- (#NaN, #Inf)/ [#cells] = (%d, %d)/[%d]\n", 0, 1, (int)o[9], (int)o[10],
+             Nxp * Nyp * Nzp);
+  }
+  printf("budgets: mass %.12e kg, rho*theta %.12e kg K\n", out[11], out[4 * 12 + 11]);
+  fflush(stdout);
+}
+
+static double output(int it) {
+  double t = wall();
+  if (ioOutputMode == 1) {
+    printf("Dumping state at timestep = %d...\n", it);
+    if (ioWriteBinaryoutFileSingleTime(it, Nxp, Nyp, Nzp, Nh) == 0) printf("Dumped state at timestep = %d...\n", it);
+  } else {
+    static int told = 0;
+    if (!told) printf("ioOutputMode = 0 (netCDF) output is not part of this build; set ioOutputMode = 1 for binary output\n");
+    told = 1;
+  }
+  return wall() - t;
 }
 
 int main(int argc, char **argv) {
@@ -45,16 +72,29 @@ int main(int argc, char **argv) {
   rc = fe_setup_device();
   if (rc) { fprintf(stderr, "device setup failed (%d)\n", rc); return 3; }
   printf("kernel variant: %s\n", fe_kernel_info());
-  double t0 = wall();
-  for (int it = simTime_it; it < Nt; it += NtBatch) {
-    double tb = wall();
+  /* main timestep loop, SRC/FEMAIN/FastEddy.c:333-396: state summary and output every
+   * frqOutput steps before the batch, then the batch and its timing table */
+  int it;
+  for (it = simTime_it; it < Nt; it += NtBatch) {
+    const double tb = wall();
+    double tio = 0.0;
+    printf("Main Time: Advancing simulation timestep = %d... and time = %10.5f\n", it, it * dt);
+    if (frqOutput > 0 && it % frqOutput == 0) {
+      state_log(it);
+      tio = output(it);
+    }
     rc = fe_run_batch();
     if (rc) { fprintf(stderr, "cuda_timeIntCommence failed (%d)\n", rc); return 4; }
-    double te = wall();
-    printf("Total Time (s) %10.4f | Batch Steps %6d | Time/step (s) %10.6f | Comp./step (s) %10.6f | step %d\n",
-           te - t0, NtBatch, (te - tb) / NtBatch, fe_last_ms_per_step() * 1e-3, simTime_it);
-    summary();
+    const double te = wall();
+    printf(" Total Time (s) | Batch Steps \t| Time/step (s) | Comp./step (s) | IO Time (s)\n");
+    printf("   %8.4f \t| %8d \t|  %8.4f \t|  %8.4f \t |  %9.6f \n", te - tb, NtBatch, (te - tb) / NtBatch,
+           (te - tb - tio) / NtBatch, tio);
+    printf("   device step time (HIP events): %.6f s\n", fe_last_ms_per_step() * 1e-3);
   }
+  /* final state summary and output (FastEddy.c:399-429) */
+  state_log(it);
+  output(it);
+  printf("Your FastEddy simulation is complete!\n");
   fe_teardown();
   return 0;
 }

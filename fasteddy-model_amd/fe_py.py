@@ -112,6 +112,7 @@ class Model:
         self.inpath = write_in(self.params, os.path.join(workdir, "case.in"))
         rc = self.L.fe_setup_host(self.inpath.encode())
         if rc != 0:
+            self.L.fe_teardown()
             raise RuntimeError(f"fe_setup_host failed with {rc}")
         self.device = False
         if device:

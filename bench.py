@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """Throughput of the FastEddy compressible-LES RK3 step on one MI355X.
 
-Workload: the dry convective boundary layer at production size, 512x512x128
+Workload (default, --workload dry): the dry convective boundary layer at production size, 512x512x128
 cells at dx=dy=dz=10 m, dt=0.02 s, FP32 state (rho, rho*u, rho*v, rho*w,
 rho*theta, rho*e), seeded uniform +/-0.25 K theta perturbations below 400 m,
 geostrophic wind (10,0) m/s, surface heat flux 0.35 K m/s, f=1e-4 1/s,
 5th-order upwind advection, 1.5-order TKE closure, RK3.  Synthetic initial
 state from the reference's own initial-condition rules; inputs resident in
-HBM before the timed region.
+HBM before the timed region.  --workload moist runs the 256x256x160 moist
+shallow-cumulus case (dx=25 m, dt=0.05 s, N_prog = 8 with rho*q_v and rho*q_l).
 
 value = whole-job cell-steps/s: interior cells x RK3 steps / wall seconds of
 exactly --steps steps, bracketed by device synchronisation.
@@ -69,25 +70,49 @@ def cpu_baseline(params, threads, nsteps=2):
                       f"untimed step (FP32 CPU oracle, OpenMP x{threads}); {dt:.1f} s timed"}
 
 
+def moist_initial_state(m, seed=2024, nh=3):
+    """Shallow-cumulus moisture: q_v = 17 g/kg at the ground, 13 g/kg at 1.5 km, 4 g/kg at 3 km,
+    seeded uniform +/-0.025 g/kg below 1.6 km, no liquid (theta perturbations of +/-0.1 K below
+    1.6 km come from the case's own initial-condition rules)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    hydro, mq, zpos = m.array("hydroFlds"), m.array("moistScalars"), m.array("zPos")
+    _, nxh, nyh, nzh = hydro.shape
+    sl = (slice(nh, nxh - nh), slice(nh, nyh - nh), slice(nh, nzh - nh))
+    z = zpos[sl].astype(np.float64)
+    qv = np.interp(z, [0.0, 1500.0, 3000.0, 1e6], [17.0, 13.0, 4.0, 4.0])
+    qv = qv + np.where(z < 1600.0, rng.uniform(-0.025, 0.025, size=qv.shape), 0.0)
+    mq[0][sl] = (hydro[0][sl].astype(np.float64) * qv).astype(mq.dtype)
+    mq[1][sl] = 0.0
+    if m.L.fe_upload_state() != 0:
+        raise RuntimeError("fe_upload_state failed")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--workload", choices=("dry", "moist"), default="dry",
+                    help="dry: the 512x512x128 CBL (the headline line); moist: the 256x256x160 shallow-cumulus case")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is a design in DESIGN.md, not part of this build)", file=sys.stderr)
+              "(multi-GPU sharding is not part of this build)", file=sys.stderr)
         sys.exit(2)
 
     import fe_py
-    params = dict(fe_py.CFG1, Nx=args.nx, Ny=args.nx, Nz=args.nz)
+    moist = args.workload == "moist"
+    base = fe_py.CFG2 if moist else fe_py.CFG1
+    params = dict(base, Nx=args.nx or base["Nx"], Ny=args.nx or base["Ny"], Nz=args.nz or base["Nz"])
     cells = params["Nx"] * params["Ny"] * params["Nz"]
-    nprog = 6
+    nprog = 8 if moist else 6
     m = fe_py.Model(params, os.path.join("/tmp", "fe_bench"))
+    if moist:
+        moist_initial_state(m)
     L = m.L
     L.fe_copy_probe.restype = C.c_double
     L.fe_copy_probe.argtypes = [C.c_long, C.c_int]
@@ -128,7 +153,7 @@ def main():
         "peak": PEAK_HBM / 1e9,
         "unit": "GB/s",
         "frac": achieved / PEAK_HBM,
-        "traffic": traffic_from_profiles("stage_kernel"),
+        "traffic": None if moist else traffic_from_profiles("stage_kernel"),
         "kernel": "fe::stage_kernel (fused tendency + RK3 stage update, 3 launches/step)",
         "avg_launch_ms": {"stage_plain": ms[1] / max(n_plain, 1), "stage_physics": ms[2] / max(n_phys, 1),
                           "surface_pass": ms[0] / max(cnt[0], 1)},
@@ -148,7 +173,9 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"plain dry CBL {params['Nx']}x{params['Ny']}x{params['Nz']} dx=10m dt=0.02s FP32 RK3 (prognostic TKE)",
+        "config": {"workload": (f"plain moist shallow cumulus {params['Nx']}x{params['Ny']}x{params['Nz']} dx=25m dt=0.05s "
+                                "FP32 RK3 (prognostic TKE, q_v, q_l, saturation adjustment)") if moist else
+                               f"plain dry CBL {params['Nx']}x{params['Ny']}x{params['Nz']} dx=10m dt=0.02s FP32 RK3 (prognostic TKE)",
                    "cells": cells, "n_prog": nprog, "kernel_variant": variant},
         "roofline": roof,
         "cpu_baseline": cpu,

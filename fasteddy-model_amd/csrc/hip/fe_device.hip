@@ -31,6 +31,8 @@ using namespace fe;
  * define.  This weak default (the well-conditioned p' form) applies when the
  * layer is linked under those modules; the host module here overrides it. */
 extern "C" __attribute__((weak)) int amd_pressureForm = 1;
+/* z segments per stage launch: 0 chooses them from the grid (see choose_segments) */
+extern "C" __attribute__((weak)) int amd_zSegments = 0;
 
 #define HIPCHK(x)                                                                               \
   do {                                                                                          \
@@ -48,6 +50,7 @@ struct Dev {
   int fecuda = 0, grid = 0, hydro = 0, timeint = 0;
   hipStream_t stream = nullptr;
   int Nx = 0, Ny = 0, Nz = 0, Nh = 0, P = 0, NF = 0, nTKE = 0, nM = 0;
+  int nseg = 1, zseg = 0;          /* z segments of the stage launches and their length */
   long fs = 0, ps = 0;
   int np = 0;
   std::vector<float> hprof;
@@ -112,8 +115,8 @@ __global__ void k_dev_to_host(const float *src, float *dst, int Nx, int Ny, int 
   float v;
   if (rule == 8 && (i < Nh || i >= Nx + Nh || j < Nh || j >= Ny + Nh || z < 0 || z >= Nz)) v = 0.f;
   else if (z >= 0 && z < Nz) v = src[z * ps + col];
-  else if (rule == 0) v = prof[PR_RHO * np + z + 3];
-  else if (rule == 4) v = prof[PR_RTH * np + z + 3];
+  else if (rule == 0) v = prof[(z + 3) * PROW + PR_RHO];
+  else if (rule == 4) v = prof[(z + 3) * PROW + PR_RTH];
   else if (rule == 7) v = (k == 0) ? 0.f : src[(z < 0 ? 0 : Nz - 1) * ps + col];
   else if (z < 0) v = src[col];
   else if (rule == 1) v = uTop;
@@ -218,6 +221,7 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
   a.slQts = surflayer_ideal_qts; a.slQte = surflayer_ideal_qte; a.slQamp = surflayer_ideal_qamp;
   a.stage = stage;
   a.emitF = emitF;
+  a.zseg = D.zseg;
   if (moistureSelector > 0 && moistureCond > 0 && moistureNvars > 1) {
     int tf = (int)roundf(fmaxf(moistureMPcallTscale, dt) / dt);
     a.mpActive = (simTime_it % tf) == 0;
@@ -225,6 +229,25 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
 }
 
 inline int physics_stage(int stage) { return (physics_oneRKonly == 0) || (stage == numRKstages); }
+
+/* Split the column into z segments, one work-group per tile and segment, when the
+ * tiles alone leave compute units idle: a stage work-group holds one compute unit
+ * (its LDS), so fewer than 256 tiles (e.g. 256 x 256 cells = 128 tiles) would run
+ * the chip at tiles/256.  Segments are even and at least 16 levels long (each costs
+ * two warm-up levels); amd_zSegments > 0 sets the count. */
+void choose_segments() {
+  const int tiles = ((D.Nx + TX - 1) / TX) * ((D.Ny + TY - 1) / TY);
+  int n = amd_zSegments;
+  if (n <= 0) {
+    n = 1;
+    while (tiles * n < 256 && D.Nz / (n + 1) >= 16) ++n;
+  }
+  n = std::max(1, std::min(n, (D.Nz + 1) / 2));
+  int len = (D.Nz + n - 1) / n;
+  len += len & 1;
+  D.zseg = len;
+  D.nseg = (D.Nz + len - 1) / len;
+}
 
 /* launch one RK stage (surface-layer pass first when the stage carries the physics) */
 int launch_stage(int stage, const float *in, const float *base, float *out, int emitF, float *pout) {
@@ -237,7 +260,7 @@ int launch_stage(int stage, const float *in, const float *base, float *out, int 
     hipLaunchKernelGGL(g_var.surface, cg, dim3(256), 0, D.stream, a);
     timed_end();
   }
-  dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY);
+  dim3 grid((D.Nx + TX - 1) / TX, (D.Ny + TY - 1) / TY, D.nseg);
   timed_begin(phys ? 2 : 1);
   /* the batch's last stage also emits the SGS stresses and tendencies the host will read */
   const bool emit = a.tau || a.mtau || a.fem;
@@ -428,8 +451,8 @@ int cuda_gridInit(void) {
           return FE_ERR_TERRAIN;
         }
       }
-  D.hprof.assign((size_t)NPROF * D.np, 0.f);
-  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)row * D.np + z + 3]; };
+  D.hprof.assign((size_t)PROW * D.np, 0.f);
+  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)(z + 3) * PROW + row]; };
   for (int z = -3; z < D.Nz + 7; ++z) {
     int k = z + Nh;
     int kc = k < 0 ? 0 : (k >= Nzp + 2 * Nh ? Nzp + 2 * Nh - 1 : k);
@@ -465,6 +488,7 @@ int cuda_hydroCoreInit(void) {
   D.nTKE = (turbulenceSelector > 0 && TKESelector > 0) ? 1 : 0;
   D.nM = moistureSelector > 0 ? moistureNvars : 0;
   D.NF = 5 + D.nTKE + D.nM;
+  choose_segments();
   /* the stage kernels address all fields of a state array through one buffer
    * resource with 32-bit offsets */
   if ((long)D.NF * D.fs * (long)sizeof(float) > 0xFFFFFFFFL) {
@@ -484,7 +508,7 @@ int cuda_hydroCoreInit(void) {
           return FE_ERR_TERRAIN;
         }
       }
-  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)row * D.np + z + 3]; };
+  auto P = [&](int row, int z) -> float & { return D.hprof[(size_t)(z + 3) * PROW + row]; };
   const float c1 = R_gas / powf(refPressure, R_cp);
   const float pio2 = (float)(0.5 * acos(-1.0));
   long rtop = idx3(i0, j0, kMax - 1);
@@ -591,7 +615,13 @@ int cuda_timeIntCleanup(void) {
 int fe_upload_state(void) { return D.hydro ? upload_state() : FE_ERR_STATE; }
 int fe_download_state(void) { return D.hydro ? download_state(0) : FE_ERR_STATE; }
 float fe_last_ms_per_step(void) { return D.ms_per_step; }
-const char *fe_kernel_info(void) { return g_var_ok ? g_var.name : "none"; }
+const char *fe_kernel_info(void) {
+  static char buf[160];
+  if (!g_var_ok) return "none";
+  if (D.nseg > 1) snprintf(buf, sizeof(buf), "%s, %d z-segments of %d levels", g_var.name, D.nseg, D.zseg);
+  else snprintf(buf, sizeof(buf), "%s", g_var.name);
+  return buf;
+}
 
 int fe_device_steps(int nsteps) {
   if (!D.timeint) return FE_ERR_STATE;

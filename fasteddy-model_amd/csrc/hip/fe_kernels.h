@@ -30,6 +30,11 @@ enum {
   PR_PBSRAW,    /* reference-form base pressure term         */
   NPROF
 };
+/* The profile table is level-major, PROW floats per level (levels -3 .. Nz+6):
+ * value (p, z) at [(z + 3) * PROW + p], so the rows of one level are contiguous
+ * and a level's scalar loads share one address. */
+constexpr int PROW = 16;
+static_assert(NPROF <= PROW, "profile rows exceed the level stride");
 
 struct SurfArgs {
   float *cd, *ch, *cq, *ust, *hf, *tsk, *iol, *z0m, *z0t, *qf, *qsk; /* [Ny][P] */
@@ -68,6 +73,7 @@ struct StepArgs {
   float slWth, slWq, slTr, slQr, slTs, slTe, slAmp, slQts, slQte, slQamp;
   int stage;
   int emitF;
+  int zseg;             /* levels per z segment (even); work-group z index = segment */
 };
 
 }  // namespace fe

@@ -46,7 +46,7 @@ __device__ __forceinline__ int wrapi(int v, int n) {
  * reading them through the constant address space makes them scalar loads,
  * which wait on their own counter instead of behind the prefetched vector loads. */
 typedef const __attribute__((address_space(4))) float cfloat;
-__device__ __forceinline__ float prof(const StepArgs &a, int p, int z) { return ((cfloat *)a.prof)[p * a.np + z + 3]; }
+__device__ __forceinline__ float prof(const StepArgs &a, int p, int z) { return ((cfloat *)a.prof)[(z + 3) * PROW + p]; }
 
 /* ---- advective face fluxes: u * f(face) from phi[i-3..i+2] (cuda_advectionDevice.cu:123-926) ---- */
 __device__ __forceinline__ float weno5_side(float a, float b, float c, float d, float e) {
@@ -464,6 +464,7 @@ template <class C>
 struct Tile {
   static constexpr int NF = C::NF;
   int tid, tx, ty, x0, y0, cx, cy, col;
+  int zs, ze;                  /* levels [zs, ze) of this work-group's z segment */
   bool active, baseIn, hasH, hasR1;
   int hlx, hly, hcol, hslot, hd, od;
   unsigned pb, colb, hcolb;
@@ -487,6 +488,8 @@ struct Tile {
     if ((nbt & 7) == 0) b = (b & 7) * (nbt >> 3) + (b >> 3);
     x0 = (b % nbx) * TX;
     y0 = (b / nbx) * TY;
+    zs = blockIdx.z * a.zseg;
+    ze = min(a.Nz, zs + a.zseg);
     active = (x0 + tx < a.Nx) && (y0 + ty < a.Ny);
     cx = wrapi(x0 + tx, a.Nx);
     cy = wrapi(y0 + ty, a.Ny);
@@ -612,12 +615,18 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   auto lvl = [&](float (&v)[NF], int zz, unsigned cb) { T.lvl(a, v, zz, cb); };
   auto mfac = [&](float rho, float qv) { return T.mfac(a, rho, qv); };
 
+  /* The loop starts at level z0: 0 for the first z segment; two levels below the
+   * segment's first level zs otherwise, those levels being computed without output so
+   * that every value carried from level to level is in place when level zs is stored. */
+  const int zs = T.zs, ze = T.ze;
+  const int z0 = zs > 0 ? zs - 2 : 0;
+
   /* register window of the own column, z-3..z+3 */
   float win[NF][7];
 #pragma unroll
   for (int j = 0; j < OD + 3; ++j) {
     float t[NF];
-    lvl(t, j - 3, colb);
+    lvl(t, z0 + j - 3, colb);
 #pragma unroll
     for (int f = 0; f < NF; ++f) win[f][j] = t[f];
   }
@@ -625,7 +634,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   float hv[PD + 1][NF];
 #pragma unroll
   for (int l = 0; l < PD; ++l) {
-    if (hasH) lvl(hv[l], l, hcolb);
+    if (hasH) lvl(hv[l], z0 + l, hcolb);
     else {
 #pragma unroll
       for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
@@ -634,10 +643,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 
   float pz[3];
   {
+    float q2 = C::NM > 0 ? win[C::IQV][2] : 0.f;
     float q3 = C::NM > 0 ? win[C::IQV][3] : 0.f, q4 = C::NM > 0 ? win[C::IQV][4] : 0.f;
-    pz[1] = pprime(a, win[4][3], mfac(win[0][3], q3), 0);
-    pz[0] = pz[1];
-    pz[2] = (Nz > 1) ? pprime(a, win[4][4], mfac(win[0][4], q4), 1) : pz[1];
+    pz[1] = pprime(a, win[4][3], mfac(win[0][3], q3), z0);
+    pz[0] = (z0 > 0) ? pprime(a, win[4][2], mfac(win[0][2], q2), z0 - 1) : pz[1];
+    pz[2] = (z0 + 1 < Nz) ? pprime(a, win[4][4], mfac(win[0][4], q4), z0 + 1) : pz[1];
   }
   float gzlo[NF];
 #pragma unroll
@@ -662,38 +672,38 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
       for (int j = 0; j < 5; ++j) th[j] = win[4][j + 1] * riz[j];     /* levels -2..2 */
       float lm, qm, l0, q0, l1, q1;
-      sgs_lq(a, -1, th[0], th[1], th[2], win[C::IE][2] * riz[1], lm, qm);
-      sgs_lq(a, 0, th[1], th[2], th[3], win[C::IE][3] * riz[2], l0, q0);
-      sgs_lq(a, 1, th[2], th[3], th[4], win[C::IE][4] * riz[3], l1, q1);
+      sgs_lq(a, z0 - 1, th[0], th[1], th[2], win[C::IE][2] * riz[1], lm, qm);
+      sgs_lq(a, z0, th[1], th[2], th[3], win[C::IE][3] * riz[2], l0, q0);
+      sgs_lq(a, z0 + 1, th[2], th[3], th[4], win[C::IE][4] * riz[3], l1, q1);
       kmo = a.c_k * 0.5f * (q0 + qm); lso = l0;
       kmo1 = a.c_k * 0.5f * (q1 + q0); lso1 = l1; qo1 = q1;
       if (hasR1) {
         float t2[NF], t1[NF];
-        lvl(t2, -2, hcolb);
-        lvl(t1, -1, hcolb);
+        lvl(t2, z0 - 2, hcolb);
+        lvl(t1, z0 - 1, hcolb);
         const float r2 = frcp(t2[0]), r1 = frcp(t1[0]), r0 = frcp(hv[0][0]), rp = frcp(hv[1][0]);
         float lq, qq, l0h, q0h;
-        sgs_lq(a, -1, t2[4] * r2, t1[4] * r1, hv[0][4] * r0, t1[C::IE] * r1, lq, qq);
-        sgs_lq(a, 0, t1[4] * r1, hv[0][4] * r0, hv[1][4] * rp, hv[0][C::IE] * r0, l0h, q0h);
+        sgs_lq(a, z0 - 1, t2[4] * r2, t1[4] * r1, hv[0][4] * r0, t1[C::IE] * r1, lq, qq);
+        sgs_lq(a, z0, t1[4] * r1, hv[0][4] * r0, hv[1][4] * rp, hv[0][C::IE] * r0, l0h, q0h);
         kh = a.c_k * 0.5f * (q0h + qq); lh = l0h; qh = q0h;
       }
     }
-    /* u,v,w of levels -1 and 0, theta and moisture of level 0, on the ring-1 region */
+    /* u,v,w of levels z0-1 and z0, theta and moisture of level z0, on the ring-1 region */
     for (int i = tid; i < DW * DH; i += NT) {
       int dx = i % DW - 1, dy = i / DW - 1;
       int c = wrapi(y0 + dy, a.Ny) * a.P + wrapi(x0 + dx, a.Nx);
 #pragma unroll
       for (int lev = -1; lev <= 0; ++lev) {
         float t[NF];
-        lvl(t, lev, (unsigned)c * 4u);
+        lvl(t, z0 + lev, (unsigned)c * 4u);
         float r = frcp(t[0]);
-        VL(lev, 0)[dy + 1][dx + 1] = t[1] * r;
-        VL(lev, 1)[dy + 1][dx + 1] = t[2] * r;
-        VL(lev, 2)[dy + 1][dx + 1] = t[3] * r;
+        VL(z0 + lev, 0)[dy + 1][dx + 1] = t[1] * r;
+        VL(z0 + lev, 1)[dy + 1][dx + 1] = t[2] * r;
+        VL(z0 + lev, 2)[dy + 1][dx + 1] = t[3] * r;
         if (lev == 0) {
-          VL(lev, 3)[dy + 1][dx + 1] = t[4] * r;
+          VL(z0, 3)[dy + 1][dx + 1] = t[4] * r;
 #pragma unroll
-          for (int m = 0; m < C::NM; ++m) VL(lev, 4 + m)[dy + 1][dx + 1] = t[C::IQV + m] * r;
+          for (int m = 0; m < C::NM; ++m) VL(z0, 4 + m)[dy + 1][dx + 1] = t[C::IQV + m] * r;
         }
       }
     }
@@ -701,12 +711,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
     Aown[1] = kmo * win[0][3];
     __syncthreads();
     {
-      auto v0 = [&](int q) -> Plane & { return VL(0, q); };
+      /* SGS fluxes through the lower face of level z0: the surface layer's at the ground
+       * (z0 = 0); for a later segment these only feed the first, unstored level */
+      auto v0 = [&](int q) -> Plane & { return VL(z0, q); };
       const int Y = ty + 1, X = tx + 1;
       float A = Aown[1];
-      float Bv = A * (1.f + 2.f * lso / prof(a, PR_DELTA, 0));
+      float Bv = A * (1.f + 2.f * lso / prof(a, PR_DELTA, z0));
       float E = C::TKE ? (2.f / 3.f) * win[C::IE][3] : 0.f;
-      float s33 = prof(a, PR_J33, 0) * a.dZi * (win[3][3] * riz[2] - win[3][2] * riz[1]);
+      float s33 = prof(a, PR_J33, z0) * a.dZi * (win[3][3] * riz[2] - win[3][2] * riz[1]);
       float s11 = a.dXi * (v0(0)[Y][X] - v0(0)[Y][X - 1]);
       float s22 = a.dYi * (v0(1)[Y][X] - v0(1)[Y - 1][X]);
       zc[0] = a.s.t13[col];
@@ -716,7 +728,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 #pragma unroll
       for (int m = 0; m < C::NM; ++m)
         zc[4 + m] = (m == 0) ? a.s.tq3[col]
-                             : -Bv * prof(a, PR_J33, 0) * a.dZi * (win[C::IQV + m][3] * riz[2] - win[C::IQV + m][2] * riz[1]);
+                             : -Bv * prof(a, PR_J33, z0) * a.dZi * (win[C::IQV + m][3] * riz[2] - win[C::IQV + m][2] * riz[1]);
     }
   }
 
@@ -725,10 +737,11 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   for (int f = 0; f < NF; ++f) bse[f] = bsn[f] = 0.f;
   if (!a.emitF && !baseIn) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) bse[f] = T.ld_bs(f, colb);
+    for (int f = 0; f < NF; ++f) bse[f] = T.ld_bs(f, (unsigned)z0 * pb + colb);
   }
 
-  for (int z = 0; z < Nz; ++z) {
+  for (int z = z0; z < ze; ++z) {
+    const bool out = active && z >= zs;      /* levels below zs are the segment's warm-up */
     /* ---- loads, in order of first use (vector-memory waits are in order): own
      *      level z+4 (window shift at the end of this level), the halo of level
      *      z+PD, the RK base of level z+1 (next level's P3) ---- */
@@ -1027,7 +1040,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             F[4] += ((thd * a.L_v) / (Tf * a.cp)) * fc;
           }
         }
-        if (EMIT && active && a.tau) {
+        if (EMIT && out && a.tau) {
           /* SGS stresses of the own cell at level z, in the reference's field order
            * (11, 21, 31, 32, 22, 33, theta 1..3; moisture x, y, z per species) */
           const float tv[9] = {t11, t21, zc[0], zc[1], t22, zc[2], tb1[3][ty][tx], tb2[3][ty][tx], zc[3]};
@@ -1048,7 +1061,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         Aown[1] = Aown[2];
       }
 
-      if (active) {
+      if (out) {
         float b0[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) b0[f] = baseIn ? win[f][3] : bse[f];
@@ -1140,15 +1153,22 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
    *   pp[S..S+3]: own p' of levels it-3..it;  ri[S..S+2]: own rho^-1 of levels it-2..it;
    *   pg[S]: horizontal p' differences of level it-2 (pg[S+1] written for it-1);
    *   gz[S]: z flux through the lower face of level it-2 (gz[S+1] its upper face). */
+  /* A z segment [zs, ze) other than the first starts its pipeline at it0 = zs - 2: P1
+   * of levels zs-2 and zs-1 and P2/P3 of level zs-1 run without output, so that p',
+   * the plane buffers and the z flux through the segment's lower face are in place
+   * when level zs is stored. */
+  const int zs = T.zs, ze = T.ze;
+  const int it0 = zs > 0 ? zs - 2 : 0;
+  const int zlo = zs > 0 ? zs - 1 : 0;                /* lowest level whose P3 runs */
   float win[NF][9];
 #pragma unroll
   for (int j = 0; j < 9; ++j) {
     float t[NF];
     if (j >= 2 && j < 7) {
-      T.lvl(a, t, j - 5, colb);
+      T.lvl(a, t, it0 + j - 5, colb);
     } else {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels -5, -4: never read; 7, 8: loaded */
+      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels it0-5, it0-4: never read; 7, 8: loaded */
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) win[f][j] = t[f];
@@ -1157,7 +1177,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     if (T.hasH && l < 2) {
-      T.lvl(a, hv[l], l, hcolb);
+      T.lvl(a, hv[l], it0 + l, hcolb);
     } else {
 #pragma unroll
       for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
@@ -1236,7 +1256,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
     }
 
     /* ---- P3: own cell of level it-2 ---- */
-    if (z3 >= 0) {
+    if (z3 >= zlo) {
       float dxy[NF], F[NF];
       float (*wv)[9] = win;
       float w7[NF][7];
@@ -1251,12 +1271,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
       }
       dyn_tendency<C, 7>(a, z3, w7, ri[S], ri[S + 1], pp[S], pp[S + 2], pg[S][0], pg[S][1], dxy, gz[S], gz[S + 1],
                          T.omb, F);
-      if (T.active) {
-        float b0[NF];
+      if (z3 >= zs) {                                 /* uniform: not a warm-up level */
+        if (T.active) {
+          float b0[NF];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3 + S] : bs[S][f];
-        rk_store<C, EMIT>(a, T, o3, F, b0);
-        if (a.pout) *(float *)((char *)a.pout + o3) = pp[S + 1];
+          for (int f = 0; f < NF; ++f) b0[f] = T.baseIn ? win[f][3 + S] : bs[S][f];
+          rk_store<C, EMIT>(a, T, o3, F, b0);
+          if (a.pout) *(float *)((char *)a.pout + o3) = pp[S + 1];
+        }
       }
     } else {
 #pragma unroll
@@ -1265,9 +1287,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
     __syncthreads();
   };
 
-  for (int it = 0; it < Nz + 2; it += 2) {
+  for (int it = it0; it < ze + 2; it += 2) {
     step(std::integral_constant<int, 0>{}, it);
-    if (it + 1 < Nz + 2) step(std::integral_constant<int, 1>{}, it + 1);
+    if (it + 1 < ze + 2) step(std::integral_constant<int, 1>{}, it + 1);
     /* ---- rotate the carried arrays by two slots ---- */
 #pragma unroll
     for (int f = 0; f < NF; ++f) {

@@ -131,6 +131,7 @@ extern float *hydroTauFlds, *moistTauFlds;
 extern float *cdFld, *chFld, *cqFld, *fricVel, *htFlux, *tskin, *invOblen, *z0m, *z0t, *qFlux, *qskin;
 /* Build-only keys (namespaced 'amd_', so reference .in files stay valid). */
 extern int amd_thetaPerturbationSeed;   /* srand() seed of the IC perturbation; the reference uses rank+12345 */
+extern int amd_zSegments;              /* z segments per stage launch, 0 = chosen from the grid */
 extern int amd_pressureForm;            /* 1 = perturbation form of p' (default), 0 = reference difference of powers */
 int hydro_coreGetParams(void);
 int hydro_coreInit(void);

@@ -1161,28 +1161,33 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
   const int it0 = zs > 0 ? zs - 2 : 0;
   const int zlo = zs > 0 ? zs - 1 : 0;                /* lowest level whose P3 runs */
   float win[NF][9];
-#pragma unroll
-  for (int j = 0; j < 9; ++j) {
-    float t[NF];
-    if (j >= 2 && j < 7) {
-      T.lvl(a, t, it0 + j - 5, colb);
-    } else {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) t[f] = 0.f;      /* levels it0-5, it0-4: never read; 7, 8: loaded */
-    }
-#pragma unroll
-    for (int f = 0; f < NF; ++f) win[f][j] = t[f];
-  }
   float hv[4][NF];
+  /* the first segment's levels are compile-time constants (their boundary rules fold) */
+  auto init = [&](int base) {
 #pragma unroll
-  for (int l = 0; l < 4; ++l) {
-    if (T.hasH && l < 2) {
-      T.lvl(a, hv[l], it0 + l, hcolb);
-    } else {
+    for (int j = 0; j < 9; ++j) {
+      float t[NF];
+      if (j >= 2 && j < 7) {
+        T.lvl(a, t, base + j - 5, colb);
+      } else {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
+        for (int f = 0; f < NF; ++f) t[f] = 0.f;    /* levels base-5, base-4: never read; 7, 8: loaded */
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) win[f][j] = t[f];
     }
-  }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      if (T.hasH && l < 2) {
+        T.lvl(a, hv[l], base + l, hcolb);
+      } else {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) hv[l][f] = 0.f;
+      }
+    }
+  };
+  if (it0 == 0) init(0);
+  else init(it0);
   float bs[3][NF], gz[3][NF];
 #pragma unroll
   for (int l = 0; l < 3; ++l)

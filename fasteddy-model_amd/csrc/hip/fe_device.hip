@@ -51,6 +51,9 @@ struct Dev {
   hipStream_t stream = nullptr;
   int Nx = 0, Ny = 0, Nz = 0, Nh = 0, P = 0, NF = 0, nTKE = 0, nM = 0;
   int nseg = 1, zseg = 0;          /* z segments of the stage launches and their length */
+  float *lsfM = nullptr;           /* slab means of rho, rho*u, rho*v, rho*theta, rho*q_v, levels -1..Nz */
+  float *lsfR = nullptr;           /* large-scale forcing rows, LSF_ROW per level */
+  bool lsfNow = false;             /* the current step's last stage carries the forcing */
   long fs = 0, ps = 0;
   int np = 0;
   std::vector<float> hprof;
@@ -271,10 +274,128 @@ int launch_stage(int stage, const float *in, const float *base, float *out, int 
   return FE_OK;
 }
 
+/* Slab means over the halo-inclusive plane (fecuda_PlugIns.cu:25-45 reduces the
+ * halo-inclusive extents and divides by their product), one work-group per level
+ * -1..Nz; halo values follow the boundary rules of the state (periodic sides; at the
+ * bottom and lid: rho, rho*theta from the base state, rho*u, rho*v zero-gradient below
+ * and geostrophic above, q_v zero-gradient).  FP64 sums. */
+__global__ void __launch_bounds__(256) k_lsf_means(const float *S, long fs, long ps, int Nx, int Ny, int Nz, int P, int Nh,
+                                                   int iqv, const float *prof, float uTop, float vTop, float *M) {
+  const int z = (int)blockIdx.x - 1;
+  const int nxh = Nx + 2 * Nh, nyh = Ny + 2 * Nh;
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int c = threadIdx.x; c < nxh * nyh; c += 256) {
+    const int j = c / nxh, i = c - j * nxh;
+    const int x = ((i - Nh) % Nx + Nx) % Nx, y = ((j - Nh) % Ny + Ny) % Ny;
+    const long col = (long)y * P + x;
+    const int zc = z < 0 ? 0 : (z >= Nz ? Nz - 1 : z);
+    const long o = (long)zc * ps + col;
+    const bool in = (z >= 0 && z < Nz);
+    acc[0] += in ? S[o] : prof[(z + 3) * PROW + PR_RHO];
+    acc[1] += in || z < 0 ? S[fs + o] : uTop;
+    acc[2] += in || z < 0 ? S[2 * fs + o] : vTop;
+    acc[3] += in ? S[4 * fs + o] : prof[(z + 3) * PROW + PR_RTH];
+    acc[4] += S[(long)iqv * fs + o];
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f)
+    for (int off = 32; off > 0; off >>= 1) acc[f] += __shfl_xor(acc[f], off);
+  __shared__ double part[4][5];
+  if (threadIdx.x % 64 == 0)
+    for (int f = 0; f < 5; ++f) part[threadIdx.x / 64][f] = acc[f];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int f = threadIdx.x;
+    const double s = part[0][f] + part[1][f] + part[2][f] + part[3][f];
+    M[f * (Nz + 2) + z + 1] = (float)(s / ((double)nxh * (double)nyh));
+  }
+}
+
+struct LsfPar { float p[3][5]; int horMn, moist; float fac; };
+
+__device__ __forceinline__ float lsf_piece(const float *pp, float z, float rho) {
+  if (z <= pp[3]) return (pp[0] + ((pp[1] - pp[0]) / pp[3]) * z) * rho;
+  if (z <= pp[4]) return (pp[1] + ((pp[2] - pp[1]) / (pp[4] - pp[3])) * (z - pp[3])) * rho;
+  return pp[2] * rho;
+}
+
+/* Forcing rows per level (cudaDevice_lsfRHS, cuda_largeScaleForcingsDevice.cu:141-250): the
+ * density is the base state's, values are per hour times the step factor, and the
+ * subsidence term carries the density twice, as the reference writes it. */
+__global__ void k_lsf_rows(LsfPar q, const float *prof, const float *M, int Nz, float *R) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= Nz) return;
+  const float rho = prof[(z + 3) * PROW + PR_RHO], zz = prof[(z + 3) * PROW + PR_Z];
+  float *r = R + (long)z * LSF_ROW;
+  for (int k = 0; k < LSF_ROW; ++k) r[k] = 0.f;
+  r[2] = (float)((double)(lsf_piece(q.p[1], zz, rho) * q.fac) / 3600.0);
+  if (q.moist) r[4] = (float)((double)(lsf_piece(q.p[2], zz, rho) * q.fac) / 3600.0);
+  if (q.horMn) {
+    const float fw = (float)((double)(lsf_piece(q.p[0], zz, rho) * q.fac) / 3600.0);
+    const float dz = prof[(z + 4) * PROW + PR_Z] - prof[(z + 2) * PROW + PR_Z];
+    const int n = Nz + 2, kp = z + 2, km = z;        /* rows of levels z+1 and z-1 */
+    auto sub = [&](int f) { return fw * rho * (M[f * n + kp] / M[kp] - M[f * n + km] / M[km]) / dz; };
+    r[0] = -sub(1);
+    r[1] = -sub(2);
+    r[3] = -sub(3);
+    r[5] = -sub(4);
+  }
+}
+
+/* The forcing enters the last stage's update phi = phi^n + dt (F + F_lsf) through its RK
+ * base: phi^n += dt F_lsf before that stage (phi^n is read there by the update only), so the
+ * stage kernels carry no forcing code and the positivity clip acts on the total as in
+ * cuda_RKschemes.cu:140-157.  The emitted tendencies of such a step leave F_lsf out. */
+__global__ void k_lsf_base(float *S, long fs, long ps, int Nx, int Ny, int Nz, int P, int iqv, int moist, float dt,
+                           const float *R) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+  if (x >= Nx) return;
+  const long o = (long)z * ps + (long)y * P + x;
+  const float *r = R + (long)z * LSF_ROW;
+  S[fs + o] += dt * r[0];
+  S[2 * fs + o] += dt * r[1];
+  S[4 * fs + o] += dt * (r[2] + r[3]);
+  if (moist) S[(long)iqv * fs + o] += dt * (r[4] + r[5]);
+}
+
+/* Large-scale forcing of the step starting at simTime_it: slab means of the step's input
+ * when the subsidence terms are on (cuda_timeIntDevice.cu:125-127) and the rows for the
+ * last stage when the forcing is due (cuda_hydroCoreDevice.cu:401-406). */
+int lsf_prepare() {
+  D.lsfNow = false;
+  if (lsfSelector != 1) return FE_OK;
+  const int itn = (int)roundf(lsf_freq / dt);
+  if (!(simTime_it > simTime_itRestart && itn > 0 && simTime_it % itn == 0)) return FE_OK;
+  if (lsf_horMnSubTerms == 1) {
+    hipLaunchKernelGGL(k_lsf_means, dim3(D.Nz + 2), dim3(256), 0, D.stream, D.S[0], D.fs, D.ps, D.Nx, D.Ny, D.Nz, D.P, D.Nh,
+                       5 + D.nTKE, D.prof, D.uTop, D.vTop, D.lsfM);
+    HIPCHK(hipGetLastError());
+  }
+  if (simTime_it - simTime_itRestart < itn) return FE_OK;
+  LsfPar q;
+  const float pw[5] = {lsf_w_surf, lsf_w_lev1, lsf_w_lev2, lsf_w_zlev1, lsf_w_zlev2};
+  const float pt[5] = {lsf_th_surf, lsf_th_lev1, lsf_th_lev2, lsf_th_zlev1, lsf_th_zlev2};
+  const float pq[5] = {lsf_qv_surf, lsf_qv_lev1, lsf_qv_lev2, lsf_qv_zlev1, lsf_qv_zlev2};
+  for (int k = 0; k < 5; ++k) { q.p[0][k] = pw[k]; q.p[1][k] = pt[k]; q.p[2][k] = pq[k]; }
+  q.horMn = lsf_horMnSubTerms == 1;
+  q.moist = D.nM > 0;
+  q.fac = (float)itn;
+  hipLaunchKernelGGL(k_lsf_rows, dim3((D.Nz + 63) / 64), dim3(64), 0, D.stream, q, D.prof, D.lsfM, D.Nz, D.lsfR);
+  HIPCHK(hipGetLastError());
+  D.lsfNow = true;
+  return FE_OK;
+}
+
 int one_step(float *pout_last) {
   int rc;
+  if ((rc = lsf_prepare())) return rc;
   if ((rc = launch_stage(0, D.S[0], D.S[0], D.S[1], 0, nullptr))) return rc;
   if ((rc = launch_stage(1, D.S[1], D.S[0], D.S[2], 0, nullptr))) return rc;
+  if (D.lsfNow) {
+    hipLaunchKernelGGL(k_lsf_base, dim3((D.Nx + 63) / 64, D.Ny, D.Nz), dim3(64), 0, D.stream, D.S[0], D.fs, D.ps, D.Nx,
+                       D.Ny, D.Nz, D.P, 5 + D.nTKE, D.nM > 0 ? 1 : 0, dt, D.lsfR);
+    HIPCHK(hipGetLastError());
+  }
   if ((rc = launch_stage(2, D.S[2], D.S[0], D.S[0], 0, pout_last))) return rc;
   return FE_OK;
 }
@@ -562,13 +683,19 @@ int cuda_hydroCoreInit(void) {
   HIPCHK(hipMalloc(&D.surf, (size_t)SF_N * D.ps * sizeof(float)));
   HIPCHK(hipMemset(D.surf, 0, (size_t)SF_N * D.ps * sizeof(float)));
   HIPCHK(hipMalloc(&D.stg, (size_t)D.n3h * sizeof(float)));
+  if (lsfSelector == 1) {
+    HIPCHK(hipMalloc(&D.lsfM, (size_t)5 * (D.Nz + 2) * sizeof(float)));
+    HIPCHK(hipMemset(D.lsfM, 0, (size_t)5 * (D.Nz + 2) * sizeof(float)));
+    HIPCHK(hipMalloc(&D.lsfR, (size_t)LSF_ROW * D.Nz * sizeof(float)));
+    HIPCHK(hipMemset(D.lsfR, 0, (size_t)LSF_ROW * D.Nz * sizeof(float)));
+  }
   D.hydro = 1;
   return FE_OK;
 }
 
 int cuda_hydroCoreCleanup(void) {
   for (int s = 0; s < 3; ++s) if (D.S[s]) { hipFree(D.S[s]); D.S[s] = nullptr; }
-  float **p[] = {&D.pout, &D.tau, &D.mtau, &D.fem, &D.surf, &D.stg};
+  float **p[] = {&D.pout, &D.tau, &D.mtau, &D.fem, &D.surf, &D.stg, &D.lsfM, &D.lsfR};
   for (auto q : p) if (*q) { hipFree(*q); *q = nullptr; }
   D.hydro = 0;
   return FE_OK;

@@ -75,6 +75,9 @@ struct StepArgs {
   int emitF;
   int zseg;             /* levels per z segment (even); work-group z index = segment */
 };
+/* large-scale forcing rows per level: subsidence of u, v; theta advection, theta
+ * subsidence; q_v advection, q_v subsidence */
+constexpr int LSF_ROW = 8;
 
 }  // namespace fe
 #endif

@@ -121,7 +121,7 @@ void *fe_host_array(const char *name, long *count) {
   X(thetaPerturbationSwitch) X(physics_oneRKonly) X(timeMethod) X(Nt) X(NtBatch) X(simTime_it) \
   X(simTime_itRestart) X(numRKstages) X(tBx) X(tBy) X(tBz) X(numProcsX) X(numProcsY) X(hydroSubGridWrite) \
   X(hydroForcingWrite) X(hydroForcingLog) X(verticalDeformSwitch) X(coordHorizHalos) X(frqOutput) \
-  X(amd_thetaPerturbationSeed) X(amd_pressureForm) X(amd_zSegments) X(diffusionSelector) X(lsfSelector)
+  X(amd_thetaPerturbationSeed) X(amd_pressureForm) X(amd_zSegments) X(diffusionSelector) X(lsfSelector) X(lsf_horMnSubTerms)
 #define FLTS(X) X(d_xi) X(d_eta) X(d_zeta) X(dX) X(dY) X(dZ) X(dXi) X(dYi) X(dZi) X(b_hyb) X(TKEAdvSelector_b_hyb) \
   X(coriolisLatitude) X(corioConstHorz) X(corioConstVert) X(corioLS_fact) X(c_s) X(c_k) X(surflayer_z0) \
   X(surflayer_z0t) X(surflayer_tr) X(surflayer_wth) X(surflayer_wq) X(surflayer_qr) X(surflayer_ideal_ts) \
@@ -130,7 +130,8 @@ void *fe_host_array(const char *name, long *count) {
   X(temp_grnd) X(pres_grnd) X(rho_grnd) X(theta_grnd) X(zStableBottom) X(stableGradient) X(zStableBottom2) \
   X(stableGradient2) X(zStableBottom3) X(stableGradient3) X(U_g) X(V_g) X(z_Ug) X(z_Vg) X(Ug_grad) X(Vg_grad) \
   X(thetaHeight) X(thetaAmplitude) X(R_gas) X(R_vapor) X(Rv_Rg) X(cp_gas) X(cv_gas) X(accel_g) X(R_cp) X(cp_R) \
-  X(cp_cv) X(refPressure) X(kappa) X(L_v) X(dt) X(simTime) X(verticalDeformFactor) X(verticalDeformQuadCoeff)
+  X(cp_cv) X(refPressure) X(kappa) X(L_v) X(dt) X(simTime) X(verticalDeformFactor) X(verticalDeformQuadCoeff) \
+  X(lsf_freq)
 
 int fe_get_int(const char *name, int *value) {
 #define X(v) if (strcmp(name, #v) == 0) { *value = v; return 0; }

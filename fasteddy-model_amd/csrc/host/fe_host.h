@@ -110,6 +110,11 @@ extern float surflayer_z0, surflayer_z0t, surflayer_tr, surflayer_wth, surflayer
 extern float surflayer_ideal_ts, surflayer_ideal_te, surflayer_ideal_amp;
 extern float surflayer_ideal_qts, surflayer_ideal_qte, surflayer_ideal_qamp;
 extern int lsfSelector;
+extern float lsf_w_surf, lsf_w_lev1, lsf_w_lev2, lsf_w_zlev1, lsf_w_zlev2;
+extern float lsf_th_surf, lsf_th_lev1, lsf_th_lev2, lsf_th_zlev1, lsf_th_zlev2;
+extern float lsf_qv_surf, lsf_qv_lev1, lsf_qv_lev2, lsf_qv_zlev1, lsf_qv_zlev2;
+extern int lsf_horMnSubTerms, lsf_numPhiVars;
+extern float lsf_freq;
 extern int moistureSelector, moistureNvars, moistureAdvSelectorQv, moistureAdvSelectorQi;
 extern int moistureSGSturb, moistureCond, moistureCondBasePres;
 extern float moistureAdvSelectorQv_b, moistureCondTscale, moistureMPcallTscale;

@@ -32,6 +32,11 @@ float surflayer_z0, surflayer_z0t, surflayer_tr, surflayer_wth, surflayer_wq, su
 float surflayer_ideal_ts, surflayer_ideal_te, surflayer_ideal_amp;
 float surflayer_ideal_qts, surflayer_ideal_qte, surflayer_ideal_qamp;
 int lsfSelector;
+float lsf_w_surf, lsf_w_lev1, lsf_w_lev2, lsf_w_zlev1, lsf_w_zlev2;
+float lsf_th_surf, lsf_th_lev1, lsf_th_lev2, lsf_th_zlev1, lsf_th_zlev2;
+float lsf_qv_surf, lsf_qv_lev1, lsf_qv_lev2, lsf_qv_zlev1, lsf_qv_zlev2;
+int lsf_horMnSubTerms, lsf_numPhiVars;
+float lsf_freq;
 int moistureSelector, moistureNvars, moistureAdvSelectorQv, moistureAdvSelectorQi;
 int moistureSGSturb, moistureCond, moistureCondBasePres;
 float moistureAdvSelectorQv_b, moistureCondTscale, moistureMPcallTscale;
@@ -119,6 +124,31 @@ int hydro_coreGetParams(void) {
   surflayer_ideal_qts = 0.0; surflayer_ideal_qte = 0.0; surflayer_ideal_qamp = 0.1;
   surflayer_stab = 0;      CHK(queryIntegerParameter("surflayer_stab", &surflayer_stab, 0, 1, PARAM_OPTIONAL));
   lsfSelector = 0;         CHK(queryIntegerParameter("lsfSelector", &lsfSelector, 0, 1, PARAM_MANDATORY));
+  /* large-scale forcing, hydro_core.c:269-310 (values per hour; heights in m) */
+  lsf_w_surf = lsf_w_lev1 = lsf_w_lev2 = 0.0f; lsf_w_zlev1 = 100.0f; lsf_w_zlev2 = 200.0f;
+  lsf_th_surf = lsf_th_lev1 = lsf_th_lev2 = 0.0f; lsf_th_zlev1 = 100.0f; lsf_th_zlev2 = 200.0f;
+  lsf_qv_surf = lsf_qv_lev1 = lsf_qv_lev2 = 0.0f; lsf_qv_zlev1 = 100.0f; lsf_qv_zlev2 = 200.0f;
+  lsf_horMnSubTerms = 0; lsf_freq = 1.0f; lsf_numPhiVars = 0;
+  if (lsfSelector > 0) {
+    CHK(queryFloatParameter("lsf_w_surf", &lsf_w_surf, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_w_lev1", &lsf_w_lev1, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_w_lev2", &lsf_w_lev2, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_w_zlev1", &lsf_w_zlev1, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_w_zlev2", &lsf_w_zlev2, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_th_surf", &lsf_th_surf, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_th_lev1", &lsf_th_lev1, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_th_lev2", &lsf_th_lev2, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_th_zlev1", &lsf_th_zlev1, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_th_zlev2", &lsf_th_zlev2, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_qv_surf", &lsf_qv_surf, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_qv_lev1", &lsf_qv_lev1, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_qv_lev2", &lsf_qv_lev2, -1e+4, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_qv_zlev1", &lsf_qv_zlev1, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_qv_zlev2", &lsf_qv_zlev2, 0.0, 1e+4, PARAM_MANDATORY));
+    CHK(queryIntegerParameter("lsf_horMnSubTerms", &lsf_horMnSubTerms, 0, 1, PARAM_MANDATORY));
+    CHK(queryFloatParameter("lsf_freq", &lsf_freq, 1e-3, 1e+3, PARAM_MANDATORY));
+    if (lsf_horMnSubTerms == 1) lsf_numPhiVars = 5;   /* rho, u, v, theta, qv */
+  }
   moistureSelector = 0;    CHK(queryIntegerParameter("moistureSelector", &moistureSelector, 0, 1, PARAM_MANDATORY));
   moistureNvars = 0; moistureAdvSelectorQv = 0; moistureAdvSelectorQv_b = 0.0; moistureAdvSelectorQi = 0;
   moistureSGSturb = 0; moistureCond = 1; moistureCondTscale = 1.0; moistureCondBasePres = 0; moistureMPcallTscale = 1.0;
@@ -171,8 +201,14 @@ int hydro_coreGetParams(void) {
   amd_zSegments = 0;
   CHK(queryIntegerParameter("amd_zSegments", &amd_zSegments, 0, 4096, PARAM_NOT_REQUIRED));
 #undef CHK
-  if (diffusionSelector != 0 || lsfSelector != 0) {
-    printf("ERROR: diffusionSelector and lsfSelector must be 0 in this build\n");
+  if (diffusionSelector != 0) {
+    printf("ERROR: diffusionSelector must be 0 in this build\n");
+    return -1;
+  }
+  /* the subsidence terms force and average q_v (cuda_largeScaleForcingsDevice.cu:127-132,242-245):
+   * the reference has no dry form of them */
+  if (lsfSelector > 0 && lsf_horMnSubTerms == 1 && moistureSelector == 0) {
+    printf("ERROR: lsf_horMnSubTerms = 1 needs moistureSelector = 1\n");
     return -1;
   }
   return rc;

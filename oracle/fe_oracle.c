@@ -129,7 +129,7 @@ void *oracle_create(const fe_cfg *cfg) {
   nf[OR_MOISTTAU] = 3 * nf[OR_MOIST]; nf[OR_FCOND] = 1;
   nf[OR_TIME0] = 5 + o->nTKE + o->nMoist;
   for (int i = 0; i < OR_NUM_ARRAYS; i++) {
-    long n = (i >= OR_CD) ? o->N2 : nf[i] * o->N3;
+    long n = (i == OR_LSFMEAN) ? 5L * o->NZH : (i >= OR_CD) ? o->N2 : nf[i] * o->N3;
     o->cnt[i] = n;
     o->arr[i] = calloc((size_t)n, sizeof(real));
     if (!o->arr[i]) { oracle_destroy(o); return NULL; }
@@ -955,6 +955,82 @@ static void filter_w(Oracle *o) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Large-scale forcing, cuda_largeScaleForcingsDevice.cu:100-252 and          */
+/* fecuda_PlugIns.cu:25-45.                                                    */
+static int lsf_itnum(const Oracle *o) { return (int)roundf((float)(o->c.lsf_freq / (float)o->dt)); }
+
+/* Slab means of rho, rho*u, rho*v, rho*theta and rho*qv at every halo-inclusive level,
+ * over the halo-inclusive plane (the reference's reduction spans the halo-inclusive
+ * extents and divides by their product); halos hold the current state's boundary values. */
+static void lsf_slab_means(Oracle *o) {
+  real *M = A(o, OR_LSFMEAN);
+  const real *h = A(o, OR_HYDRO), *q = A(o, OR_MOIST);
+  const real *src[5] = {h + RHO * o->N3, h + U * o->N3, h + V * o->N3, h + TH * o->N3, q};
+  for (int f = 0; f < 5; f++)
+    for (int k = 0; k < o->NZH; k++) {
+      double acc = 0.0;
+      for (int i = 0; i < o->NXH; i++)
+        for (int j = 0; j < o->NYH; j++) acc += (double)src[f][ID3(o, i, j, k)];
+      M[f * o->NZH + k] = (real)(acc / ((double)o->NXH * (double)o->NYH));
+    }
+}
+
+/* piecewise-linear profile of (surf, lev1, lev2, zlev1, zlev2) at height z, times rho */
+static real lsf_piece(const double *pp, real z, real rho) {
+  const real surf = (real)pp[0], lev1 = (real)pp[1], lev2 = (real)pp[2], z1 = (real)pp[3], z2 = (real)pp[4];
+  if (z <= z1) return (surf + ((lev1 - surf) / z1) * z) * rho;
+  if (z <= z2) return (lev1 + ((lev2 - lev1) / (z2 - z1)) * (z - z1)) * rho;
+  return lev2 * rho;
+}
+
+/* cudaDevice_lsfRHS: the density is the base state's (the launch at
+ * cuda_hydroCoreDevice.cu:405 passes hydroBaseStateFlds_d), the forcing is given per
+ * hour and applied every round(lsf_freq/dt) steps with that factor; the subsidence
+ * term carries the density twice, as the reference writes it. */
+static void lsf_rhs(Oracle *o, real fac) {
+  const real *rb = A(o, OR_BS), *zp = A(o, OR_ZPOS), *M = A(o, OR_LSFMEAN);
+  real *F = A(o, OR_HYDRO_F), *Fq = A(o, OR_MOIST_F);
+  const int moist = o->c.moistureSelector > 0;
+  const long NZH = o->NZH, N3 = o->N3;
+  for (int i = o->iMin; i < o->iMax; i++)
+    for (int j = o->jMin; j < o->jMax; j++)
+      for (int k = o->kMin; k < o->kMax; k++) {
+        const long n = ID3(o, i, j, k);
+        const real rho = rb[n], z = zp[n];
+        real fth = lsf_piece(o->c.lsf_th, z, rho);
+        fth = (real)((double)(fth * fac) / 3600.0);
+        F[TH * N3 + n] = F[TH * N3 + n] + fth;
+        if (moist) {
+          real fqv = lsf_piece(o->c.lsf_qv, z, rho);
+          fqv = (real)((double)(fqv * fac) / 3600.0);
+          Fq[n] = Fq[n] + fqv;
+        }
+        if (o->c.lsf_horMnSubTerms == 1) {
+          real fw = lsf_piece(o->c.lsf_w, z, rho);
+          fw = (real)((double)(fw * fac) / 3600.0);
+          const real dz = zp[n + 1] - zp[n - 1];
+          const real *R = M, *Um = M + NZH, *Vm = M + 2 * NZH, *Tm = M + 3 * NZH, *Qm = M + 4 * NZH;
+          F[U * N3 + n] = F[U * N3 + n] - fw * rho * (Um[k + 1] / R[k + 1] - Um[k - 1] / R[k - 1]) / dz;
+          F[V * N3 + n] = F[V * N3 + n] - fw * rho * (Vm[k + 1] / R[k + 1] - Vm[k - 1] / R[k - 1]) / dz;
+          F[TH * N3 + n] = F[TH * N3 + n] - fw * rho * (Tm[k + 1] / R[k + 1] - Tm[k - 1] / R[k - 1]) / dz;
+          Fq[n] = Fq[n] - fw * rho * (Qm[k + 1] / R[k + 1] - Qm[k - 1] / R[k - 1]) / dz;
+        }
+      }
+}
+
+/* Slab means of the input of the step starting at simTime_it, when the subsidence
+ * terms are on and the forcing step comes round (cuda_timeIntDevice.cu:125-127). */
+int oracle_lsf_prepare(void *h, int simTime_it, int simTime_itRestart) {
+  Oracle *o = (Oracle *)h;
+  if (o->c.lsfSelector == 1 && o->c.lsf_horMnSubTerms == 1 && simTime_it > simTime_itRestart &&
+      simTime_it % lsf_itnum(o) == 0) {
+    oracle_apply_bcs(o);
+    lsf_slab_means(o);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 int oracle_build_frhs(void *h, float simTime, int simTime_it, int simTime_itRestart, int stage) {
   Oracle *o = (Oracle *)h;
   const fe_cfg *c = &o->c;
@@ -1039,6 +1115,12 @@ int oracle_build_frhs(void *h, float simTime, int simTime_it, int simTime_itRest
     }
   }
 
+  /* 6. large-scale forcing in the last stage every round(lsf_freq/dt) steps (:400-406) */
+  if (c->lsfSelector == 1 && stage == numRKstages && simTime_it > simTime_itRestart) {
+    const int itn = lsf_itnum(o);
+    if (simTime_it - simTime_itRestart >= itn && simTime_it % itn == 0) lsf_rhs(o, (real)(float)itn);
+  }
+
   /* 7. explicit vertical filter (:414-416) */
   if (c->filterSelector > 0) filter_w(o);
   return 0;
@@ -1080,6 +1162,7 @@ int oracle_rk_stage(void *h, int stage) {
 int oracle_steps(void *h, int nsteps, float *simTime, int *simTime_it, int simTime_itRestart) {
   Oracle *o = (Oracle *)h;
   for (int s = 0; s < nsteps; s++) {
+    oracle_lsf_prepare(o, *simTime_it, simTime_itRestart);
     for (int st = 0; st < 3; st++) {
       oracle_build_frhs(o, *simTime, *simTime_it, simTime_itRestart, st);
       oracle_rk_stage(o, st);

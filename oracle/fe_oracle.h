@@ -54,6 +54,10 @@ typedef struct {
   double U_g, V_g, z_Ug, z_Vg, Ug_grad, Vg_grad;
   int physics_oneRKonly;
   double dt;
+  /* large-scale forcing (hydro_core.c:267-310): surf, lev1, lev2, zlev1, zlev2 of w, theta, qv */
+  int lsfSelector, lsf_horMnSubTerms;
+  double lsf_w[5], lsf_th[5], lsf_qv[5];
+  double lsf_freq;
 } fe_cfg;
 
 /* Array identifiers for oracle_array(). */
@@ -77,6 +81,7 @@ enum {
   /* 2-D surface arrays */
   OR_CD, OR_CH, OR_CQ, OR_FRICVEL, OR_HTFLUX, OR_TSKIN, OR_INVOBLEN,
   OR_Z0M, OR_Z0T, OR_QFLUX, OR_QSKIN,
+  OR_LSFMEAN,       /* 5 slab-mean profiles (rho, rho*u, rho*v, rho*theta, rho*qv), NZH each */
   OR_NUM_ARRAYS
 };
 
@@ -91,6 +96,7 @@ int   oracle_build_frhs(void *h, float simTime, int simTime_it, int simTime_itRe
 /* One RK3-WS2002 stage update using the current F. */
 int   oracle_rk_stage(void *h, int stage);
 /* nsteps full RK3 steps (3 x (build_frhs + rk_stage)); advances *simTime and *simTime_it. */
+int   oracle_lsf_prepare(void *h, int simTime_it, int simTime_itRestart);
 int   oracle_steps(void *h, int nsteps, float *simTime, int *simTime_it, int simTime_itRestart);
 /* Apply the stage-start boundary conditions to the state arrays only. */
 int   oracle_apply_bcs(void *h);

@@ -43,12 +43,17 @@ class FeCfg(C.Structure):
         ("Ug_grad", C.c_double), ("Vg_grad", C.c_double),
         ("physics_oneRKonly", C.c_int),
         ("dt", C.c_double),
+        ("lsfSelector", C.c_int), ("lsf_horMnSubTerms", C.c_int),
+        ("lsf_w", C.c_double * 5), ("lsf_th", C.c_double * 5), ("lsf_qv", C.c_double * 5),
+        ("lsf_freq", C.c_double),
     ]
+
+LSF_PARTS = ("surf", "lev1", "lev2", "zlev1", "zlev2")
 
 
 OR_IDS = ["HYDRO", "HYDRO_F", "TKE", "TKE_F", "MOIST", "MOIST_F", "BS", "ZPOS", "J31", "J32", "J33", "DJAC",
           "INVDJAC", "PRES", "RHOINV", "FACEVEL", "TAU", "KM", "LS", "DEDXI", "MOISTTAU", "FCOND", "TIME0",
-          "CD", "CH", "CQ", "FRICVEL", "HTFLUX", "TSKIN", "INVOBLEN", "Z0M", "Z0T", "QFLUX", "QSKIN"]
+          "CD", "CH", "CQ", "FRICVEL", "HTFLUX", "TSKIN", "INVOBLEN", "Z0M", "Z0T", "QFLUX", "QSKIN", "LSFMEAN"]
 OR = {n: i for i, n in enumerate(OR_IDS)}
 
 _DEFAULTS = {"physics_oneRKonly": 1, "pgfSelector": 1, "buoyancySelector": 1, "surflayer_stab": 0,
@@ -63,6 +68,15 @@ def make_cfg(params):
     """fe_cfg from a parameter dict; float parameters are rounded to float32 as sscanf('%f') does."""
     cfg = FeCfg()
     for name, ctype in FeCfg._fields_:
+        if name in ("lsf_w", "lsf_th", "lsf_qv"):
+            arr = getattr(cfg, name)
+            for i, part in enumerate(LSF_PARTS):
+                dflt = 100.0 if part == "zlev1" else 200.0 if part == "zlev2" else 0.0
+                arr[i] = float(np.float32(params.get(f"{name}_{part}", dflt)))
+            continue
+        if name == "lsf_freq":
+            cfg.lsf_freq = float(np.float32(params.get("lsf_freq", 1.0)))
+            continue
         v = params.get(name, _DEFAULTS.get(name, 0))
         if ctype is C.c_double:
             setattr(cfg, name, float(np.float32(v)))
@@ -89,6 +103,7 @@ class Oracle:
             L.oracle_rk_stage.argtypes = [C.c_void_p, C.c_int]
             L.oracle_steps.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_int]
             L.oracle_apply_bcs.argtypes = [C.c_void_p]
+            L.oracle_lsf_prepare.argtypes = [C.c_void_p, C.c_int, C.c_int]
             L.oracle_set_threads.argtypes = [C.c_int]
             Oracle._libs[bits] = L
         self.L = Oracle._libs[bits]
@@ -113,6 +128,8 @@ class Oracle:
         ct = C.c_float if self.bits == 32 else C.c_double
         flat = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(n.value,))
         n3 = self.shape3[0] * self.shape3[1] * self.shape3[2]
+        if name == "LSFMEAN":
+            return flat.reshape((5, self.shape3[2]))
         if n.value >= n3 and n.value % n3 == 0:
             return flat.reshape((n.value // n3,) + self.shape3)
         return flat.reshape(self.shape3[:2])
@@ -136,6 +153,10 @@ class Oracle:
         self.simTime = m.getf("simTime")
         self.it = m.geti("simTime_it")
         self.itR = m.geti("simTime_itRestart")
+
+    def lsf_prepare(self):
+        """Slab means of the current state for the large-scale forcing of step self.it."""
+        self.L.oracle_lsf_prepare(self.h, self.it, self.itR)
 
     def build_frhs(self, stage):
         self.L.oracle_build_frhs(self.h, C.c_float(self.simTime), self.it, self.itR, int(stage))

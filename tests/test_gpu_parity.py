@@ -510,3 +510,56 @@ def test_z_segments_are_bitwise_identical(name, params, moist, segs, tmp_path):
             if k[0] in "FEM":
                 levels = set(int(i[-1]) - NH for i in np.argwhere(d > 0))
                 assert levels <= starts, (nseg, k, sorted(levels), sorted(starts))
+
+
+LSF_STRONG = dict(lsfSelector=1, lsf_freq=0.05,
+                  lsf_w_surf=0.0, lsf_w_lev1=-4000.0, lsf_w_lev2=-1000.0, lsf_w_zlev1=150.0, lsf_w_zlev2=300.0,
+                  lsf_th_surf=-300.0, lsf_th_lev1=-200.0, lsf_th_lev2=100.0, lsf_th_zlev1=120.0, lsf_th_zlev2=320.0,
+                  lsf_qv_surf=-150.0, lsf_qv_lev1=-100.0, lsf_qv_lev2=50.0, lsf_qv_zlev1=100.0, lsf_qv_zlev2=250.0)
+
+
+@pytest.mark.parametrize("moist", [True, False], ids=["moist_subsidence", "dry"])
+def test_large_scale_forcing_matches_oracle(moist, tmp_path):
+    """Large-scale forcing (lsfSelector = 1; with the subsidence terms of the slab means in the
+    moist case) through three RK3 steps against the FP64 oracle, with the forcing due every
+    step after the first (lsf_freq = dt).  The forcing is made strong enough that the oracle's
+    state with it differs from the state without it by far more than the GPU-oracle difference,
+    so the comparison tests the forcing itself."""
+    if moist:
+        p = dict(fe_py.CFG2, Nx=32, Ny=16, Nz=16, d_zeta=25.0, dt=0.05, NtBatch=1, lsf_horMnSubTerms=1, **LSF_STRONG)
+    else:
+        p = dict(fe_py.CFG0, Nx=48, Ny=16, Nz=20, dt=0.05, NtBatch=1, lsf_horMnSubTerms=0, **LSF_STRONG)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=17)
+        if moist:
+            fe_py.moist_state(m, seed=18)
+        o64 = oracle_py.Oracle(p, 64)
+        o64.load_from_model(m)
+        q = dict(p, lsfSelector=0)
+        ono = oracle_py.Oracle(q, 64)
+        ono.load_from_model(m)
+        checks = [("hydroFlds", "HYDRO")] + ([("moistScalars", "MOIST")] if moist else [])
+        start = {h: m.interior(m.array(h)).astype(np.float64).copy() for h, _ in checks}
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        for _ in range(3):
+            m.commence()
+        o64.steps(3)
+        ono.steps(3)
+        for hname, oname in checks:
+            g = m.interior(m.array(hname)).astype(np.float64)
+            o = m.interior(o64.arr(oname)[: g.shape[0]])
+            n = m.interior(ono.arr(oname)[: g.shape[0]])
+            scale = np.abs(o).max(axis=(1, 2, 3))
+            change = np.abs(o - start[hname]).max(axis=(1, 2, 3))
+            diff = np.abs(g - o).max(axis=(1, 2, 3))
+            effect = np.abs(o - n).max(axis=(1, 2, 3))
+            print(f"{hname}: |gpu-fp64| {diff}, forcing effect {effect}, change {change}")
+            assert np.all(diff <= 2e-3 * change + 1e-6 * scale), (hname, diff, change)
+            forced = [4] if hname == "hydroFlds" else [0]        # rho*theta; rho*q_v
+            for f in forced:
+                assert effect[f] >= 50 * diff[f], (hname, f, effect[f], diff[f])
+        o64.close()
+        ono.close()
+    finally:
+        m.close()

@@ -748,3 +748,67 @@ def test_surface_layer_moist_neutral_transfer(tmp_path):
     rho1 = h[0][NH:-NH, NH:-NH, NH]
     np.testing.assert_allclose(o.arr("MOISTTAU")[2][NH:-NH, NH:-NH, NH], cq * U1 * 4.0 * rho1, rtol=1e-12)
     o.close()
+
+
+LSF_BOMEX = dict(lsfSelector=1, lsf_horMnSubTerms=1, lsf_freq=0.1,
+                 lsf_w_surf=0.0, lsf_w_lev1=-23.4, lsf_w_lev2=0.0, lsf_w_zlev1=1500.0, lsf_w_zlev2=2100.0,
+                 lsf_th_surf=-0.0833, lsf_th_lev1=-0.0833, lsf_th_lev2=0.0, lsf_th_zlev1=1500.0, lsf_th_zlev2=2500.0,
+                 lsf_qv_surf=-0.0432, lsf_qv_lev1=-0.0432, lsf_qv_lev2=0.0, lsf_qv_zlev1=300.0, lsf_qv_zlev2=500.0)
+
+
+def _piece(surf, lev1, lev2, z1, z2, z):
+    return np.where(z <= z1, surf + (lev1 - surf) * z / z1,
+                    np.where(z <= z2, lev1 + (lev2 - lev1) * (z - z1) / (z2 - z1), lev2))
+
+
+def test_large_scale_forcing_profiles_and_subsidence(tmp_path):
+    """BOMEX-type forcing (cuda_largeScaleForcingsDevice.cu:141-250) on the moist base state at
+    rest with q_v = q0 + G z: every other tendency vanishes (geostrophic, hydrostatic, e = 0, no
+    surface flux, no condensation below saturation), so the last stage's tendencies are, with
+    rho the base-state density, f(z) the piecewise-linear input per hour and k = lsf_freq/dt:
+      F_rho*theta = k/3600 rho f_th(z) - k/3600 rho^2 f_w(z) dtheta_BS/dz,
+      F_rho*q_v   = k/3600 rho f_qv(z) - k/3600 rho^2 f_w(z) G,
+      F_rho*u     = -k/3600 rho^2 f_w(z) d<u>/dz = 0 (uniform u),
+    the vertical derivatives being those of the slab means, exact centred differences of the
+    piecewise-linear profiles away from their kinks."""
+    dz = 25.0
+    p = dict(fe_py.CFG2, Nx=8, Ny=8, Nz=100, d_zeta=dz, dt=0.05, thetaPerturbationSwitch=0, surflayerSelector=0,
+             coriolisSelector=0, dampingLayerSelector=0, moistureCond=3, **LSF_BOMEX)
+    m = host_model(p, tmp_path)
+    o = oracle_py.Oracle(p, 64)
+    o.load_from_model(m)
+    m.close()
+    h = o.arr("HYDRO")
+    h[0] = o.arr("BS")[0]
+    h[4] = o.arr("BS")[1]
+    h[1] = p["U_g"] * h[0]
+    h[2] = 0.0
+    h[3] = 0.0
+    o.arr("TKE")[0] = 0.0
+    z = o.arr("ZPOS")[0]
+    q0, G = 5.0, -0.001
+    o.arr("MOIST")[0] = h[0] * (q0 + G * z)
+    o.arr("MOIST")[1] = 0.0
+    rho = o.arr("BS")[0]
+    th = o.arr("BS")[1] / rho
+    # step 2 of a run from step 0 with lsf_freq = 2 dt: the forcing is due; slab means of the
+    # step's input, then the last stage's tendencies
+    o.it, o.itR = 2, 0
+    o.lsf_prepare()
+    o.build_frhs(2)
+    k = 2.0 / 3600.0
+    f32 = lambda v: float(np.float32(v))
+    fth = _piece(*(f32(LSF_BOMEX[f"lsf_th_{s}"]) for s in ("surf", "lev1", "lev2", "zlev1", "zlev2")), z)
+    fqv = _piece(*(f32(LSF_BOMEX[f"lsf_qv_{s}"]) for s in ("surf", "lev1", "lev2", "zlev1", "zlev2")), z)
+    fw = _piece(*(f32(LSF_BOMEX[f"lsf_w_{s}"]) for s in ("surf", "lev1", "lev2", "zlev1", "zlev2")), z)
+    dthdz = (np.roll(th, -1, axis=2) - np.roll(th, 1, axis=2)) / (2 * dz)
+    F = o.arr("HYDRO_F")
+    Fq = o.arr("MOIST_F")[0]
+    kk = slice(NH + 1, NH + p["Nz"] - 1)
+    ex_th = k * rho * fth - k * rho * fw * rho * dthdz
+    ex_qv = k * rho * fqv - k * rho * fw * rho * G
+    inner = (slice(NH, -NH), slice(NH, -NH), kk)
+    np.testing.assert_allclose(F[4][inner], ex_th[inner], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(Fq[inner], ex_qv[inner], rtol=1e-9, atol=1e-12)
+    assert np.abs(F[1][inner]).max() < 1e-12
+    o.close()

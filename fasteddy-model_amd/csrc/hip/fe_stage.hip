@@ -477,7 +477,7 @@ struct Tile {
   unsigned fsb;
   float omb[NF];               /* (1 - b) of the hybrid schemes, 0 for the others */
 
-  __device__ __forceinline__ explicit Tile(const StepArgs &a) {
+  __device__ __forceinline__ explicit Tile(const StepArgs &a, bool r1_late = false) {
     tid = threadIdx.x;
     tx = tid % TX;
     ty = tid / TX;
@@ -500,12 +500,24 @@ struct Tile {
     rout = __builtin_amdgcn_make_buffer_rsrc((void *)a.out, 0, -1, FE_RSRC_W3);
     fsb = (unsigned)a.fs * 4u;
     pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
-    /* this thread's halo cell */
+    /* this thread's halo cell h: by default h = tid, so the ring-1 cells (h < NR1) fall on
+     * waves 0-2; with r1_late they fall on waves 2, 3 and a third of wave 6, away from the
+     * waves 0 and 1 that compute the ring faces in the plain stage, which evens the extra
+     * work over the four SIMDs (waves w and w+4 share SIMD w) */
     hlx = 0; hly = 0; hcol = 0; hslot = -1;
     hasH = tid < NHALO;
-    hasR1 = tid < NR1;
+    int h = tid;
+    if (r1_late) {
+      static_assert(NR1 == 148 && NHALO == 468, "the permutation below is laid out for the 64x8 tile");
+      if (tid >= 128 && tid < 256) h = tid - 128;
+      else if (tid >= 384 && tid < 384 + (NR1 - 128)) h = 128 + (tid - 384);
+      else if (tid < 128) h = NR1 + tid;
+      else if (tid < 384) h = NR1 + 128 + (tid - 256);
+      else h = NR1 + 256 + (tid - 384 - (NR1 - 128));
+    }
+    hasR1 = hasH && h < NR1;
     if (hasH) {
-      halo_cell(tid, hlx, hly);
+      halo_cell(h, hlx, hly);
       hslot = (hly + HH) * RW + (hlx + HH);
       hcol = wrapi(y0 + hly, a.Ny) * a.P + wrapi(x0 + hlx, a.Nx);
     }
@@ -1139,7 +1151,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
   __shared__ float gx[2][NF][TY][TX + 1];
   __shared__ float gy[2][NF][TY + 1][TX];
 
-  const Tile<C> T(a);
+  const Tile<C> T(a, true);
   const int tx = T.tx, ty = T.ty, wave = T.ty, Nz = a.Nz;
   const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
 

@@ -477,7 +477,8 @@ struct Tile {
   unsigned fsb;
   float omb[NF];               /* (1 - b) of the hybrid schemes, 0 for the others */
 
-  __device__ __forceinline__ explicit Tile(const StepArgs &a, bool r1_late = false) {
+  struct R1Layout { int a, b, c; };
+  __device__ __forceinline__ explicit Tile(const StepArgs &a, R1Layout L1 = {0, 64, 128}) {
     tid = threadIdx.x;
     tx = tid % TX;
     ty = tid / TX;
@@ -500,20 +501,23 @@ struct Tile {
     rout = __builtin_amdgcn_make_buffer_rsrc((void *)a.out, 0, -1, FE_RSRC_W3);
     fsb = (unsigned)a.fs * 4u;
     pb = (unsigned)a.ps * 4u;              /* bytes per xy plane */
-    /* this thread's halo cell h: by default h = tid, so the ring-1 cells (h < NR1) fall on
-     * waves 0-2; with r1_late they fall on waves 2, 3 and a third of wave 6, away from the
-     * waves 0 and 1 that compute the ring faces in the plain stage, which evens the extra
-     * work over the four SIMDs (waves w and w+4 share SIMD w) */
+    /* this thread's halo cell h: the ring-1 cells (h < NR1 = 148) go to the threads
+     * [r1a, r1a+64), [r1b, r1b+64) and [r1c, r1c+20), the other halo cells to the remaining
+     * threads below NHALO in thread order; where the ring-1 work lands decides how evenly the
+     * extra work of a level spreads over the four SIMDs (waves w and w+4 share SIMD w) */
     hlx = 0; hly = 0; hcol = 0; hslot = -1;
     hasH = tid < NHALO;
-    int h = tid;
-    if (r1_late) {
-      static_assert(NR1 == 148 && NHALO == 468, "the permutation below is laid out for the 64x8 tile");
-      if (tid >= 128 && tid < 256) h = tid - 128;
-      else if (tid >= 384 && tid < 384 + (NR1 - 128)) h = 128 + (tid - 384);
-      else if (tid < 128) h = NR1 + tid;
-      else if (tid < 384) h = NR1 + 128 + (tid - 256);
-      else h = NR1 + 256 + (tid - 384 - (NR1 - 128));
+    int h;
+    {
+      static_assert(NR1 == 148 && NHALO == 468, "the ring-1 layout below is for the 64x8 tile");
+      const int r1a = L1.a, r1b = L1.b, r1c = L1.c;
+      if (tid >= r1a && tid < r1a + 64) h = tid - r1a;
+      else if (tid >= r1b && tid < r1b + 64) h = 64 + tid - r1b;
+      else if (tid >= r1c && tid < r1c + 20) h = 128 + tid - r1c;
+      else {
+        const int below = min(max(tid - r1a, 0), 64) + min(max(tid - r1b, 0), 64) + min(max(tid - r1c, 0), 20);
+        h = NR1 + tid - below;
+      }
     }
     hasR1 = hasH && h < NR1;
     if (hasH) {
@@ -617,7 +621,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
   typedef float Plane[DH][DW];
   auto VL = [&](int lev, int q) -> Plane & { return q < 3 ? velm[lev & 3][q] : vels[(lev + 3) % 3][q - 3]; };
 
-  const Tile<C> T(a);
+  const Tile<C> T(a, {128, 192, 384});    /* ring-1 cells on waves 2, 3 and part of 6 */
   const int tid = T.tid, tx = T.tx, ty = T.ty, wave = T.ty;
   const int x0 = T.x0, y0 = T.y0, cx = T.cx, col = T.col, Nz = a.Nz;
   const bool active = T.active, baseIn = T.baseIn, hasH = T.hasH, hasR1 = T.hasR1;
@@ -836,8 +840,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float V = hyD * (raw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + raw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
         FaceLoop<C, 0>::yf(raw, gy, ty, tx, V, omb);
       }
-      /* ring faces, half of the fields per wave, on the waves without ring-1 cells
-       * (those carry the ring's rho^-1, p', velocities and Km in the other phases) */
+      /* ring faces, half of the fields per wave, on waves 3-6 (a placement chosen by
+       * measurement together with the ring-1 cells on waves 2, 3 and 6) */
       constexpr int HF = NF / 2;
       switch (wave) {
         case 3: ring_faces_y<C, 0, HF>(raw, gy, dr, tx, hyD, omb); break;
@@ -1151,7 +1155,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FE_PLAI
   __shared__ float gx[2][NF][TY][TX + 1];
   __shared__ float gy[2][NF][TY + 1][TX];
 
-  const Tile<C> T(a, true);
+  const Tile<C> T(a, {128, 192, 384});    /* ring-1 cells on waves 2, 3 and part of 6 */
   const int tx = T.tx, ty = T.ty, wave = T.ty, Nz = a.Nz;
   const unsigned pb = T.pb, colb = T.colb, hcolb = T.hcolb;
 

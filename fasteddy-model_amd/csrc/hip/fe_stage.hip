@@ -840,14 +840,15 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         float V = hyD * (raw[2][ty + HH - 1][tx + HH] * dr[ty][tx + 1] + raw[2][ty + HH][tx + HH] * dr[ty + 1][tx + 1]);
         FaceLoop<C, 0>::yf(raw, gy, ty, tx, V, omb);
       }
-      /* ring faces, half of the fields per wave, on waves 3-6 (a placement chosen by
-       * measurement together with the ring-1 cells on waves 2, 3 and 6) */
+      /* ring faces, half of the fields per wave: y on waves 0, 1, x on waves 4, 5 (the
+       * placement of all the ring work, with the ring-1 cells on waves 2, 3 and 6, was chosen
+       * by measurement) */
       constexpr int HF = NF / 2;
       switch (wave) {
-        case 3: ring_faces_y<C, 0, HF>(raw, gy, dr, tx, hyD, omb); break;
-        case 4: ring_faces_y<C, HF, NF>(raw, gy, dr, tx, hyD, omb); break;
-        case 5: ring_faces_x<C, 0, HF>(raw, gx, dr, tx, hxD, omb); break;
-        case 6: ring_faces_x<C, HF, NF>(raw, gx, dr, tx, hxD, omb); break;
+        case 0: ring_faces_y<C, 0, HF>(raw, gy, dr, tx, hyD, omb); break;
+        case 1: ring_faces_y<C, HF, NF>(raw, gy, dr, tx, hyD, omb); break;
+        case 4: ring_faces_x<C, 0, HF>(raw, gx, dr, tx, hxD, omb); break;
+        case 5: ring_faces_x<C, HF, NF>(raw, gx, dr, tx, hxD, omb); break;
         default: break;
       }
       if constexpr (PH) {
@@ -931,14 +932,14 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           auto eown = [&](int lx, int ly) { return raw[C::IE][ly + HH][lx + HH] * dr[ly + 1][lx + 1]; };
           qx[ty][tx + 1] = 0.5f * a.dXi * (eown(tx + 1, ty) - eown(tx - 1, ty)) * kmp[0][ty + 1][tx + 1];
           qy[ty + 1][tx] = 0.5f * a.dYi * (eown(tx, ty + 1) - eown(tx, ty - 1)) * kmp[0][ty + 1][tx + 1];
-          if (wave == 0) {
+          if (wave == 3) {
             qy[0][tx] = qyf(tx, -1);
             if (tx < 2 * TY) {
               int ly = tx % TY, side = tx / TY;
               if (side == 0) qx[ly][0] = qxf(-1, ly);
               else qx[ly][TX + 1] = qxf(TX, ly);
             }
-          } else if (wave == 1) {
+          } else if (wave == 6) {
             qy[TY + 1][tx] = qyf(tx, TY);
           }
         }

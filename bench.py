@@ -21,8 +21,8 @@ the output): 32 x N_prog = 192 B per cell-step in total.  achieved = those
 bytes over the kernel's own HIP-event durations inside the timed region.
 
 cpu_baseline: the repository's FP32 CPU oracle (a restatement of the
-reference step, OpenMP over i within each phase) timed on this host for two
-RK3 steps of the same workload (after one untimed step).
+reference step, OpenMP over i within each phase) timed on this host for six
+RK3 steps of the same workload (after one untimed step; about 10 s on 16 cores).
 """
 import argparse
 import ctypes as C
@@ -47,7 +47,7 @@ def traffic_from_profiles(kernel_tag):
         return json.load(fh).get("bytes_per_launch")
 
 
-def cpu_baseline(params, threads, nsteps=2):
+def cpu_baseline(params, threads, nsteps=6):
     """The FP32 CPU oracle on the full workload: one untimed RK3 step (first touch), then
     nsteps timed RK3 steps."""
     import fe_py

@@ -81,12 +81,12 @@ struct Dev {
 
 void timed_begin(int cls) {
   if (!D.timing || D.evn + 2 > D.ev.size()) return;
-  hipEventRecord(D.ev[D.evn], D.stream);
+  (void)hipEventRecord(D.ev[D.evn], D.stream);
   D.evcls[D.evn / 2] = cls;
 }
 void timed_end(void) {
   if (!D.timing || D.evn + 2 > D.ev.size()) return;
-  hipEventRecord(D.ev[D.evn + 1], D.stream);
+  (void)hipEventRecord(D.ev[D.evn + 1], D.stream);
   D.evn += 2;
 }
 
@@ -186,6 +186,8 @@ int select_variant(void) {
   else if (tke && nm == 0 && sh == 4 && se == 4) { g_var = make_variant<true, 0, 4, 4, 3, 5>("dry+tke weno3"); g_var_ok = 1; }
   else if (tke && nm == 0 && sh == 3 && se == 0) { g_var = make_variant<true, 0, 3, 0, 3, 5>("dry+tke hyb56/upstream"); g_var_ok = 1; }
   else if (!tke && turbulenceSelector == 0 && nm == 0 && sh == 3) { g_var = make_variant<false, 0, 3, 3, 3, 5>("dry inviscid hyb56"); g_var_ok = 1; }
+  else if (!tke && turbulenceSelector > 0 && nm == 0 && sh == 3) { g_var = make_variant<false, 0, 3, 3, 3, 5>("dry smagorinsky hyb56"); g_var_ok = 1; }
+  else if (!tke && turbulenceSelector > 0 && nm == 2 && sh == 3 && sqv == 3 && sql == 5) { g_var = make_variant<false, 2, 3, 3, 3, 5>("moist2 smagorinsky hyb56/hyb56/weno5"); g_var_ok = 1; }
   return g_var_ok ? FE_OK : FE_ERR_UNSUPPORTED;
 }
 
@@ -217,6 +219,9 @@ void fill_args(StepArgs &a, int stage, const float *in, const float *base, float
   a.c_k = c_k; a.kappa = kappa; a.refP = refPressure; a.presg = pres_grnd; a.R_cp = R_cp; a.L_v = L_v; a.cp = cp_gas;
   a.R_vapor = R_vapor; a.condTs = moistureCondTscale; a.condSel = moistureCond; a.condBase = moistureCondBasePres;
   a.mSGS = moistureSGSturb > 0;
+  /* Smagorinsky constant of the non-TKE closure; 0 without turbulence, so that the
+   * closure's K_m, and with it every SGS stress, is zero in the inviscid form */
+  a.c_s = (turbulenceSelector > 0 && TKESelector == 0) ? c_s : 0.f;
   a.step_it = simTime_it; a.step_itR = simTime_itRestart; a.simTime = simTime; a.dt = dt;
   a.slSel = surflayerSelector; a.slIdeal = surflayer_idealsine; a.slStab = surflayer_stab;
   a.slWth = surflayer_wth; a.slWq = surflayer_wq; a.slTr = surflayer_tr; a.slQr = surflayer_qr;
@@ -532,7 +537,7 @@ int fecuda_Init(void) {
 }
 
 int fecuda_Cleanup(void) {
-  if (D.stream) { hipStreamDestroy(D.stream); D.stream = nullptr; }
+  if (D.stream) { (void)hipStreamDestroy(D.stream); D.stream = nullptr; }
   D.fecuda = 0;
   return FE_OK;
 }
@@ -584,7 +589,7 @@ int cuda_gridInit(void) {
     P(PR_INVD, z) = invD_Jac[r];
     P(PR_DELTA, z) = powf(dX * dY * dZ * D_Jac[r], (float)(1.0 / 3.0));
   }
-  if (D.prof) hipFree(D.prof);
+  if (D.prof) (void)hipFree(D.prof);
   HIPCHK(hipMalloc(&D.prof, D.hprof.size() * sizeof(float)));
   HIPCHK(hipMemcpy(D.prof, D.hprof.data(), D.hprof.size() * sizeof(float), hipMemcpyHostToDevice));
   D.grid = 1;
@@ -592,7 +597,7 @@ int cuda_gridInit(void) {
 }
 
 int cuda_gridCleanup(void) {
-  if (D.prof) { hipFree(D.prof); D.prof = nullptr; }
+  if (D.prof) { (void)hipFree(D.prof); D.prof = nullptr; }
   D.grid = 0;
   return FE_OK;
 }
@@ -662,7 +667,7 @@ int cuda_hydroCoreInit(void) {
   }
   HIPCHK(hipMemcpy(D.prof, D.hprof.data(), D.hprof.size() * sizeof(float), hipMemcpyHostToDevice));
   for (int s = 0; s < 3; ++s) {
-    if (D.S[s]) hipFree(D.S[s]);
+    if (D.S[s]) (void)hipFree(D.S[s]);
     HIPCHK(hipMalloc(&D.S[s], (size_t)D.NF * D.fs * sizeof(float)));
     HIPCHK(hipMemset(D.S[s], 0, (size_t)D.NF * D.fs * sizeof(float)));
   }
@@ -694,9 +699,9 @@ int cuda_hydroCoreInit(void) {
 }
 
 int cuda_hydroCoreCleanup(void) {
-  for (int s = 0; s < 3; ++s) if (D.S[s]) { hipFree(D.S[s]); D.S[s] = nullptr; }
+  for (int s = 0; s < 3; ++s) if (D.S[s]) { (void)hipFree(D.S[s]); D.S[s] = nullptr; }
   float **p[] = {&D.pout, &D.tau, &D.mtau, &D.fem, &D.surf, &D.stg, &D.lsfM, &D.lsfR};
-  for (auto q : p) if (*q) { hipFree(*q); *q = nullptr; }
+  for (auto q : p) if (*q) { (void)hipFree(*q); *q = nullptr; }
   D.hydro = 0;
   return FE_OK;
 }
@@ -729,8 +734,8 @@ int cuda_timeIntCommence(int it) {
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   D.ms_per_step = ms / NtBatch;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return download_state(1);
 }
 
@@ -765,7 +770,7 @@ int fe_device_steps(int nsteps) {
 /* Per-kernel timing: enable with a pool for npairs launches; read the summed
  * milliseconds and launch counts per kernel class after a synchronising call. */
 int fe_timing_enable(int npairs) {
-  for (auto e : D.ev) hipEventDestroy(e);
+  for (auto e : D.ev) (void)hipEventDestroy(e);
   D.ev.clear();
   D.evcls.clear();
   D.evn = 0;
@@ -801,27 +806,32 @@ __global__ void k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ 
  * box's measured copy bandwidth. */
 double fe_copy_probe(long nfloats, int reps) {
   nfloats = (nfloats + 3) / 4 * 4;
+  if (nfloats <= 0 || reps <= 0) return -1;
   if (D.probe_n < nfloats) {
-    if (D.probe) hipFree(D.probe);
+    if (D.probe) (void)hipFree(D.probe);
+    D.probe = nullptr;
+    D.probe_n = 0;
     if (hipMalloc(&D.probe, (size_t)2 * nfloats * sizeof(float)) != hipSuccess) return -1;
-    hipMemset(D.probe, 0, (size_t)2 * nfloats * sizeof(float));
+    if (hipMemset(D.probe, 0, (size_t)2 * nfloats * sizeof(float)) != hipSuccess) return -1;
     D.probe_n = nfloats;
   }
-  if (!D.stream) hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking);
+  if (!D.stream && hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking) != hipSuccess) return -1;
   const long n4 = nfloats / 4;
   const float4 *a = (const float4 *)D.probe;
   float4 *b = (float4 *)(D.probe + nfloats);
   int blocks = 256 * 8;
   hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, D.stream, a, b, n4);
   hipEvent_t e0, e1;
-  hipEventCreate(&e0); hipEventCreate(&e1);
-  hipEventRecord(e0, D.stream);
-  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, D.stream, a, b, n4);
-  hipEventRecord(e1, D.stream);
-  hipEventSynchronize(e1);
+  if (hipEventCreate(&e0) != hipSuccess) return -1;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return -1; }
   float ms = 0.f;
-  hipEventElapsedTime(&ms, e0, e1);
-  hipEventDestroy(e0); hipEventDestroy(e1);
+  bool ok = hipEventRecord(e0, D.stream) == hipSuccess;
+  for (int r = 0; r < reps && ok; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, D.stream, a, b, n4);
+  ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, D.stream) == hipSuccess &&
+       hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.f;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!ok) return -1;
   return (double)2 * nfloats * sizeof(float) * reps / (ms * 1e-3) / 1e9;
 }
 
@@ -905,7 +915,8 @@ int fe_profiles(double *out, int nstat) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)D.Nz * PROF_NSTAT * sizeof(double), hipMemcpyDeviceToHost, D.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
-  hipFree(d);
+  const hipError_t ef = hipFree(d);
+  if (e == hipSuccess) e = ef;
   HIPCHK(e);
   /* SGS columns on cell centres, as fetutorialfunctions.py:426-429 (the top level stays 0) */
   for (int c = 14; c < 17; ++c) {
@@ -1018,7 +1029,8 @@ int fe_state_summary(double *out, int nrow) {
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(lv.data(), d, lv.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(D.stream);
-  hipFree(d);
+  const hipError_t ef = hipFree(d);
+  if (e == hipSuccess) e = ef;
   HIPCHK(e);
   const double NONE = (double)0x7fffffffffffffffLL;
   const double ncell = (double)D.Nx * D.Ny * D.Nz;

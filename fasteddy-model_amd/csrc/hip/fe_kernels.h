@@ -64,7 +64,7 @@ struct StepArgs {
   float filt;           /* 0.015625*coeff/dt, 0 when the filter is off */
   int damp;
   float uTop, vTop;     /* momentum in the top halo (geostrophic base state) */
-  float c_k, kappa, refP, presg, R_cp, L_v, cp, R_vapor, condTs;
+  float c_k, c_s, kappa, refP, presg, R_cp, L_v, cp, R_vapor, condTs;
   int condSel, condBase, mpActive, mpTheta;
   int mSGS;              /* SGS moisture fluxes on (moistureSGSturb) */
   int step_it, step_itR;

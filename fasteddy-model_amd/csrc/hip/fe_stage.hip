@@ -620,6 +620,29 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
 
   typedef float Plane[DH][DW];
   auto VL = [&](int lev, int q) -> Plane & { return q < 3 ? velm[lev & 3][q] : vels[(lev + 3) % 3][q - 3]; };
+  /* Smagorinsky eddy viscosity (cuda_sgsTurbDevice.cu:238-248, strain rates :60-197) of the
+   * cell at plane position (X, Y) of level lev (X, Y >= 1: its strain needs the cells below
+   * and to its low sides):
+   *   K_m = (c_s Delta)^2 sqrt(max(0, S11^2/2 + S22^2/2 + S33^2/2 + S21^2 + S31^2 + S32^2
+   *                                    - max(0, 3 N^2))),  N^2 = g/theta dtheta/dz */
+  auto smag = [&](int lev, int X, int Y) -> float {
+    const Plane &u = VL(lev, 0), &v = VL(lev, 1), &w = VL(lev, 2), &th = VL(lev, 3);
+    const Plane &um = VL(lev - 1, 0), &vm = VL(lev - 1, 1), &wm = VL(lev - 1, 2), &thm = VL(lev - 1, 3);
+    const float J33 = prof(a, PR_J33, lev), dz = J33 * a.dZi;
+    const float s11 = a.dXi * (u[Y][X] - u[Y][X - 1]);
+    const float s22 = a.dYi * (v[Y][X] - v[Y - 1][X]);
+    const float s33 = dz * (w[Y][X] - wm[Y][X]);
+    const float s21 = 0.25f * (a.dXi * ((v[Y][X] - v[Y][X - 1]) + (v[Y - 1][X] - v[Y - 1][X - 1]))
+                             + a.dYi * ((u[Y][X] - u[Y - 1][X]) + (u[Y][X - 1] - u[Y - 1][X - 1])));
+    const float s31 = 0.25f * (a.dXi * ((w[Y][X] - w[Y][X - 1]) + (wm[Y][X] - wm[Y][X - 1]))
+                             + dz * ((u[Y][X] - um[Y][X]) + (u[Y][X - 1] - um[Y][X - 1])));
+    const float s32 = 0.25f * (a.dYi * ((w[Y][X] - w[Y - 1][X]) + (wm[Y][X] - wm[Y - 1][X]))
+                             + dz * ((v[Y][X] - vm[Y][X]) + (v[Y - 1][X] - vm[Y - 1][X])));
+    const float t1 = 0.5f * (s11 * s11 + s22 * s22 + s33 * s33) + s21 * s21 + s31 * s31 + s32 * s32;
+    const float N2 = a.g / th[Y][X] * (dz * (th[Y][X] - thm[Y][X]));
+    const float cd = a.c_s * prof(a, PR_DELTA, lev);
+    return cd * cd * sqrtf(fmaxf(0.f, t1 - fmaxf(0.f, N2 * 3.f)));
+  };
 
   const Tile<C> T(a, {128, 192, 384});    /* ring-1 cells on waves 2, 3 and part of 6 */
   const int tid = T.tid, tx = T.tx, ty = T.ty, wave = T.ty;
@@ -716,16 +739,20 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         VL(z0 + lev, 0)[dy + 1][dx + 1] = t[1] * r;
         VL(z0 + lev, 1)[dy + 1][dx + 1] = t[2] * r;
         VL(z0 + lev, 2)[dy + 1][dx + 1] = t[3] * r;
+        VL(z0 + lev, 3)[dy + 1][dx + 1] = t[4] * r;       /* level z0-1: the Smagorinsky N^2 */
         if (lev == 0) {
-          VL(z0, 3)[dy + 1][dx + 1] = t[4] * r;
 #pragma unroll
           for (int m = 0; m < C::NM; ++m) VL(z0, 4 + m)[dy + 1][dx + 1] = t[C::IQV + m] * r;
         }
       }
     }
     Aown[0] = 0.f;
-    Aown[1] = kmo * win[0][3];
     __syncthreads();
+    if constexpr (!C::TKE) {                        /* Smagorinsky: K_m of level z0 from the planes */
+      kmo = smag(z0, tx + 1, ty + 1);
+      lso = prof(a, PR_DELTA, z0);
+    }
+    Aown[1] = kmo * win[0][3];
     {
       /* SGS fluxes through the lower face of level z0: the surface layer's at the ground
        * (z0 = 0); for a later segment these only feed the first, unstored level */
@@ -739,7 +766,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
       float s22 = a.dYi * (v0(1)[Y][X] - v0(1)[Y - 1][X]);
       zc[0] = a.s.t13[col];
       zc[1] = a.s.t23[col];
-      zc[2] = -2.f * A * s33 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+      zc[2] = -2.f * A * s33 + (C::TKE ? (2.f / 3.f) * A * (s11 + s22 + s33) : 0.f) + E;
       zc[3] = a.s.th3[col];
 #pragma unroll
       for (int m = 0; m < C::NM; ++m)
@@ -810,6 +837,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         for (int m = 0; m < C::NM; ++m) (&vb(4 + m)[0][0])[hd] = hv[1][C::IQV + m] * r;
       }
       const float idl = 2.f * frcp(prof(a, PR_DELTA, z));
+      if constexpr (!C::TKE) {
+        /* Smagorinsky: the ring-1 cells whose faces the tile uses (x = TX, y = TY; the
+         * low-side ring cells take no stress here) from the planes of levels z, z-1; the own
+         * cell's K_m of level z was formed one level earlier */
+        kh = (hasR1 && T.hlx >= 0 && T.hly >= 0) ? smag(z, T.hlx + 1, T.hly + 1) : 0.f;
+        lh = lso;
+      }
       {
         float A = win[0][3] * kmo;
         (&kmp[0][0][0])[od] = A;
@@ -866,7 +900,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
                              + a.dYi * ((vz(0)[Y][X] - vz(0)[Y - 1][X]) + (vz(0)[Y][X - 1] - vz(0)[Y - 1][X - 1])));
           float s31 = 0.25f * (a.dXi * ((vz(2)[Y][X] - vz(2)[Y][X - 1]) + (vm(2)[Y][X] - vm(2)[Y][X - 1]))
                              + J33z * a.dZi * ((vz(0)[Y][X] - vm(0)[Y][X]) + (vz(0)[Y][X - 1] - vm(0)[Y][X - 1])));
-          tb1[0][fy][fx] = -2.f * A * s11 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+          tb1[0][fy][fx] = -2.f * A * s11 + (C::TKE ? (2.f / 3.f) * A * (s11 + s22 + s33) : 0.f) + E;
           tb1[1][fy][fx] = -2.f * A * s21;
           tb1[2][fy][fx] = (z == 0) ? a.s.t13[c2] : -2.f * A * s31;
           tb1[3][fy][fx] = -Bv * a.dXi * (vz(3)[Y][X] - vz(3)[Y][X - 1]);
@@ -884,7 +918,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           float s32 = 0.25f * (a.dYi * ((vz(2)[Y][X] - vz(2)[Y - 1][X]) + (vm(2)[Y][X] - vm(2)[Y - 1][X]))
                              + J33z * a.dZi * ((vz(1)[Y][X] - vm(1)[Y][X]) + (vz(1)[Y - 1][X] - vm(1)[Y - 1][X])));
           tb2[0][fy][fx] = -2.f * A * s21;
-          tb2[1][fy][fx] = -2.f * A * s22 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+          tb2[1][fy][fx] = -2.f * A * s22 + (C::TKE ? (2.f / 3.f) * A * (s11 + s22 + s33) : 0.f) + E;
           tb2[2][fy][fx] = (z == 0) ? a.s.t23[c2] : -2.f * A * s32;
           tb2[3][fy][fx] = -Bv * a.dYi * (vz(3)[Y][X] - vz(3)[Y - 1][X]);
 #pragma unroll
@@ -900,7 +934,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           float s33 = J33z * a.dZi * (vz(2)[Y][X] - vm(2)[Y][X]);
           float s21 = 0.25f * (a.dXi * ((vz(1)[Y][X] - vz(1)[Y][X - 1]) + (vz(1)[Y - 1][X] - vz(1)[Y - 1][X - 1]))
                              + a.dYi * ((vz(0)[Y][X] - vz(0)[Y - 1][X]) + (vz(0)[Y][X - 1] - vz(0)[Y - 1][X - 1])));
-          const float iso = (2.f / 3.f) * A * (s11 + s22 + s33);
+          const float iso = C::TKE ? (2.f / 3.f) * A * (s11 + s22 + s33) : 0.f;
           t11 = -2.f * A * s11 + iso + E;
           t22 = -2.f * A * s22 + iso + E;
           t21 = -2.f * A * s21;
@@ -963,6 +997,10 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
         auto vp = [&](int q) -> const Plane & { return VL(z + 1, q); };
         auto vm = [&](int q) -> const Plane & { return VL(z - 1, q); };
         float zn[4 + C::NM];
+        if constexpr (!C::TKE) {
+          kmo1 = smag(z + 1, X, Y);
+          lso1 = prof(a, PR_DELTA, z + 1);
+        }
         {
           float A = kmo1 * win[0][4];
           float Bv = A * fmaf(2.f * lso1, frcp(prof(a, PR_DELTA, z + 1)), 1.f);
@@ -976,7 +1014,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
                              + J33p * a.dZi * ((vp(1)[Y][X] - vz(1)[Y][X]) + (vp(1)[Y - 1][X] - vz(1)[Y - 1][X])));
           zn[0] = -2.f * A * s31;
           zn[1] = -2.f * A * s32;
-          zn[2] = -2.f * A * s33 + (2.f / 3.f) * A * (s11 + s22 + s33) + E;
+          zn[2] = -2.f * A * s33 + (C::TKE ? (2.f / 3.f) * A * (s11 + s22 + s33) : 0.f) + E;
           zn[3] = -Bv * J33p * a.dZi * (vp(3)[Y][X] - vz(3)[Y][X]);
 #pragma unroll
           for (int m = 0; m < C::NM; ++m) zn[4 + m] = -Bv * J33p * a.dZi * (vp(4 + m)[Y][X] - vz(4 + m)[Y][X]);
@@ -1108,6 +1146,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
           sgs_lq(a, z + 1, hv[0][4] * h0, hv[1][4] * h1, hv[2][4] * h2, hv[1][C::IE] * h1, l1h, q1h);
           kh = a.c_k * 0.5f * (q1h + qh); lh = l1h; qh = q1h;
         }
+      } else {
+        kmo = kmo1; lso = lso1;
       }
     }
 

@@ -357,6 +357,8 @@ VARIANTS = [
     ("dry+tke weno3", dict(advectionSelector=4, TKEAdvSelector=4), False),
     ("dry+tke hyb56/upstream", dict(advectionSelector=3, TKEAdvSelector=0, b_hyb=0.5), False),
     ("dry inviscid hyb56", dict(turbulenceSelector=0, TKESelector=0, surflayerSelector=0), False),
+    ("dry smagorinsky hyb56", dict(TKESelector=0), False),
+    ("moist2 smagorinsky hyb56/hyb56/weno5", dict(TKESelector=0), True),
     ("moist2+tke hyb56/.../upstream", dict(moistureAdvSelectorQi=0), True),
     ("moist1+tke hyb56", dict(moistureNvars=1), True),
 ]
@@ -400,6 +402,8 @@ OPTIONS = [
     ("moist condensation form 1", dict(moistureCond=1, moistureCondBasePres=0), True),
     ("moist condensation form 2", dict(moistureCond=2), True),
     ("moist without SGS moisture fluxes", dict(moistureSGSturb=0), True),
+    ("Smagorinsky closure", dict(TKESelector=0), False),
+    ("moist Smagorinsky closure", dict(TKESelector=0), True),
 ]
 
 
@@ -418,7 +422,8 @@ def test_rk3_steps_options(name, over, moist, tmp_path):
         o64.load_from_model(m)
         o32 = oracle_py.Oracle(p, 32)
         o32.load_from_model(m)
-        checks = [("hydroFlds", "HYDRO"), ("sgstkeScalars", "TKE")] + ([("moistScalars", "MOIST")] if moist else [])
+        checks = [("hydroFlds", "HYDRO")] + ([("sgstkeScalars", "TKE")] if p["TKESelector"] > 0 else [])
+        checks += [("moistScalars", "MOIST")] if moist else []
         start = {h: m.interior(m.array(h)).astype(np.float64).copy() for h, _ in checks}
         m.L.fe_set_int(b"simTime_itRestart", 0)
         for _ in range(3):

@@ -300,6 +300,48 @@ def test_surface_layer_unstable_businger_dyer(tmp_path):
     o.close()
 
 
+def test_smagorinsky_eddy_viscosity_closed_form():
+    """Smagorinsky closure (TKESelector = 0) on u = S z, w = W z, theta = 300 + G z:
+    S33 = W, S31 = S/2, the other strain rates 0, so
+    Km = (c_s Delta)^2 sqrt(max(0, W^2/2 + S^2/4 - max(0, 3 g G / theta))), with the
+    unstable branch (G < 0) free of the buoyancy term and a strongly stable column at Km = 0;
+    tau_13 = -2 rho Km S/2 and the theta flux -rho Km 3 G (Prandtl number 1/3)."""
+    p = small(Nx=8, Ny=8, Nz=12, TKESelector=0, surflayerSelector=0, coriolisSelector=0,
+              d_xi=10.0, d_eta=10.0, d_zeta=10.0)
+    f32 = lambda v: float(np.float32(v))
+    cs, g, delta = f32(p["c_s"]), f32(9.81), 1000.0 ** (1.0 / 3.0)
+    S, W = 0.02, 0.004
+    nz = p["Nz"] + 2 * NH
+    z = (np.arange(nz) - NH + 0.5) * 10.0
+    ks = slice(NH + 2, NH + p["Nz"] - 2)          # clear of the ground and lid boundary rules
+    for G in (-0.01, 0.0, 1e-6, 0.1):
+        o = oracle_py.Oracle(p, 64)
+        for a in ("J33", "DJAC", "INVDJAC"):
+            o.arr(a)[...] = 1.0
+        o.arr("ZPOS")[0] = z[None, None, :]
+        h = o.arr("HYDRO")
+        h[0] = 1.0
+        h[1] = S * z[None, None, :]
+        h[2] = 0.0
+        h[3] = W * z[None, None, :]
+        h[4] = 300.0 + G * z[None, None, :]
+        o.arr("BS")[0] = 1.0
+        o.arr("BS")[1] = h[4]
+        o.build_frhs(2)
+        th = (300.0 + G * z)[ks]
+        t1 = 0.5 * W * W + 0.25 * S * S
+        expect = (cs * delta) ** 2 * np.sqrt(np.maximum(0.0, t1 - np.maximum(0.0, 3.0 * g * G / th)))
+        km = o.arr("KM").reshape(h[0].shape)[NH:-NH, NH:-NH, ks]
+        np.testing.assert_allclose(km, np.broadcast_to(expect, km.shape), rtol=1e-9, atol=1e-15)
+        tau = o.arr("TAU")
+        np.testing.assert_allclose(tau[2][NH:-NH, NH:-NH, ks], -2.0 * km * S / 2, rtol=1e-9, atol=1e-15)
+        # theta differences of G dz against theta ~ 300 K: the FP64 cancellation bounds rtol
+        np.testing.assert_allclose(tau[8][NH:-NH, NH:-NH, ks], -km * 3.0 * G, rtol=1e-6, atol=1e-18)
+        if G == 0.1:
+            assert np.all(km == 0.0)
+        o.close()
+
+
 def test_sgs_length_scale_branches_and_shear_stress():
     """e = 0 in a stable column gives l = 0.01 m; a neutral column gives l = Delta; pure shear
     u = alpha z with uniform e gives tau_13 = -2 rho Km alpha/2 with Km = c_k Delta sqrt(e)."""

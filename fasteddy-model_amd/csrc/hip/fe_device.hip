@@ -648,6 +648,7 @@ int cuda_hydroCoreInit(void) {
     P(PR_IRTH, z) = 1.0f / tbs[r];
     P(PR_PBS, z) = hydroBaseStatePres[r];
     P(PR_PBSRAW, z) = powf(tbs[r] * c1, cp_cv);
+    P(PR_EXB, z) = powf(P(PR_PBSRAW, z) / refPressure, R_cp);
     float zz = zPos[r];
     float ubs = (zz < z_Ug) ? U_g * rbs[r] : (U_g + Ug_grad * (zz - z_Ug)) * rbs[r];
     float vbs = (zz < z_Vg) ? V_g * rbs[r] : (V_g + Vg_grad * (zz - z_Vg)) * rbs[r];

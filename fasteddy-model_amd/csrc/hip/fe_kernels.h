@@ -28,6 +28,7 @@ enum {
   PR_DAMP,      /* 0.2*sin^2(...) inside the damping layer   */
   PR_DELTA,     /* (dX*dY*dZ*D)^(1/3)                        */
   PR_PBSRAW,    /* reference-form base pressure term         */
+  PR_EXB,       /* (PR_PBSRAW / refPressure)^(R/cp): Exner factor of the base pressure */
   NPROF
 };
 /* The profile table is level-major, PROW floats per level (levels -3 .. Nz+6):

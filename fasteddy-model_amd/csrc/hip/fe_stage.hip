@@ -1068,8 +1068,13 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             const float rth = win[4][3];
             const float thd = rth * rin;
             const float c1 = a.c1p;
-            float pr = a.condBase ? powf(prof(a, PR_RTH, z) * c1, a.gam) : powf(rth * c1, a.gam);
-            float Td = thd * powf(pr / a.refP, a.R_cp);
+            /* pressure and Exner factor of the cell (the thermal-forcing temperature below);
+             * with the base-pressure option the condensation uses the level's base-state
+             * pressure and its Exner factor, tabulated per level */
+            const float prf = powf(rth * c1, a.gam);
+            const float exf = powf(prf / a.refP, a.R_cp);
+            float pr = a.condBase ? prof(a, PR_PBSRAW, z) : prf;
+            float Td = thd * (a.condBase ? prof(a, PR_EXB, z) : exf);
             float Tc = fmaxf(-80.f, Td - 273.16f);
             float pvs;
             if (Td >= 273.15f)
@@ -1090,8 +1095,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(C::PHYS
             float fc = (fmaxf(ffor, 0.f) - fmaxf(fminf(flim, -ffor), 0.f)) / a.condTs;
             F[C::IQV] -= fc * 1e3f;
             F[C::IQL] += fc * 1e3f;
-            float prf = powf(rth * c1, a.gam);
-            float Tf = thd * powf(prf / a.refP, a.R_cp);
+            float Tf = thd * exf;
             F[4] += ((thd * a.L_v) / (Tf * a.cp)) * fc;
           }
         }

@@ -176,6 +176,7 @@ int select_variant(void) {
   g_var_ok = 0;
   if (tke && nm == 0 && sh == 3 && se == 3) { g_var = make_variant<true, 0, 3, 3, 3, 5>("dry+tke hyb56/hyb56"); g_var_ok = 1; }
   else if (tke && nm == 2 && sh == 3 && se == 3 && sqv == 3 && sql == 5) { g_var = make_variant<true, 2, 3, 3, 3, 5>("moist2+tke hyb56/hyb56/hyb56/weno5"); g_var_ok = 1; }
+  else if (tke && nm == 2 && sh == 3 && se == 3 && sqv == 2 && sql == 5) { g_var = make_variant<true, 2, 3, 3, 2, 5>("moist2+tke hyb56/hyb56/hyb34/weno5"); g_var_ok = 1; }
   else if (tke && nm == 2 && sh == 3 && se == 3 && sqv == 3 && sql == 0) { g_var = make_variant<true, 2, 3, 3, 3, 0>("moist2+tke hyb56/.../upstream"); g_var_ok = 1; }
   else if (tke && nm == 1 && sh == 3 && se == 3 && sqv == 3) { g_var = make_variant<true, 1, 3, 3, 3, 5>("moist1+tke hyb56"); g_var_ok = 1; }
   else if (tke && nm == 0 && sh == 5 && se == 5) { g_var = make_variant<true, 0, 5, 5, 3, 5>("dry+tke weno5/weno5"); g_var_ok = 1; }

@@ -1,0 +1,73 @@
+"""The reference's example cases on the GPU (tutorials/examples/Example0*.in: neutral, convective
+and stable boundary layers and BOMEX; parameter values in tests/golden/example_params.json).
+
+Every parameter of the example is kept except the 4-rank decomposition and the BOMEX netCDF
+initial file: one rank and a 40 x 24 horizontal grid (partial tiles in x and y), with the
+example's vertical grid (stretched for the neutral and convective cases), time step, advection
+schemes, surface layer, damping, filter, Coriolis, moisture and large-scale forcing.  Smooth
+seeded perturbations (and moisture for BOMEX) are added to the example's initial state, three
+RK3 steps run on the GPU and on the FP64 oracle, and the state must agree to 2e-3 of its change
+plus 1e-6 of its scale, as in the option tests.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fe_py
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = json.load(open(os.path.join(HERE, "golden", "example_params.json")))
+VARIANT = {
+    "Example01_NBL.in": "dry+tke hyb56/hyb56",
+    "Example02_CBL.in": "dry+tke hyb56/hyb56",
+    "Example03_SBL.in": "dry+tke hyb56/hyb56",
+    "Example04_BOMEX.in": "moist2+tke hyb56/hyb56/hyb34/weno5",
+}
+
+
+def example(name, tmp_path):
+    p = {}
+    for k, v in CASES[name]:
+        if k == "Description" or v == "":
+            continue
+        p[k] = v
+    p.update(numProcsX=1, numProcsY=1, Nx=40, Ny=24, NtBatch=1, outPath=str(tmp_path) + "/")
+    p.pop("inFile", None)
+    p.pop("inPath", None)
+    return p
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_reference_example_steps_match_oracle(name, tmp_path):
+    p = example(name, tmp_path)
+    moist = int(p.get("moistureSelector", "0")) > 0
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        assert m.L.fe_kernel_info().decode().startswith(VARIANT[name])
+        fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=3)
+        if moist:
+            fe_py.moist_state(m, seed=4)
+        o64 = oracle_py.Oracle(p, 64)
+        o64.load_from_model(m)
+        checks = [("hydroFlds", "HYDRO"), ("sgstkeScalars", "TKE")] + ([("moistScalars", "MOIST")] if moist else [])
+        start = {h: m.interior(m.array(h)).astype(np.float64).copy() for h, _ in checks}
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        for _ in range(3):
+            m.commence()
+        o64.steps(3)
+        for hname, oname in checks:
+            g = m.interior(m.array(hname)).astype(np.float64)
+            o = m.interior(o64.arr(oname)[: g.shape[0]])
+            scale = np.abs(o).max(axis=(1, 2, 3))
+            change = np.abs(o - start[hname]).max(axis=(1, 2, 3))
+            diff = np.abs(g - o).max(axis=(1, 2, 3))
+            print(f"{name} {hname}: |gpu-fp64| {diff} vs |change| {change}")
+            assert np.all(np.isfinite(g))
+            assert np.all(diff <= 2e-3 * change + 1e-6 * scale), (hname, diff, change)
+        o64.close()
+    finally:
+        m.close()

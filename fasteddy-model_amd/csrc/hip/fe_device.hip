@@ -244,18 +244,42 @@ inline int physics_stage(int stage) { return (physics_oneRKonly == 0) || (stage 
  * (its LDS), so fewer than 256 tiles (e.g. 256 x 256 cells = 128 tiles) would run
  * the chip at tiles/256.  Segments are even and at least 16 levels long (each costs
  * two warm-up levels); amd_zSegments > 0 sets the count. */
+/* Even segment length for n segments of Nz levels, and the segment count it gives. */
+static void seg_shape(int Nz, int n, int &len, int &nseg) {
+  len = (Nz + n - 1) / n;
+  len += len & 1;
+  nseg = (Nz + len - 1) / len;
+}
+
+/* A work-group holds a compute unit (its LDS), so a launch runs in rounds of one work-group
+ * per CU; a segment also computes 2 warm-up levels below its first stored level.  The count
+ * minimises rounds x (segment length + 2) over segments of at least 16 levels (fewest
+ * segments on ties): a count whose work-groups overflow the CUs by a few would add a nearly
+ * empty round. */
 void choose_segments() {
   const int tiles = ((D.Nx + TX - 1) / TX) * ((D.Ny + TY - 1) / TY);
+  static int ncu = 0;
+  if (ncu <= 0) {
+    int dev = 0;
+    hipDeviceProp_t pr;
+    ncu = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess && pr.multiProcessorCount > 0)
+              ? pr.multiProcessorCount : 256;
+  }
   int n = amd_zSegments;
   if (n <= 0) {
+    long best = -1;
     n = 1;
-    while (tiles * n < 256 && D.Nz / (n + 1) >= 16) ++n;
+    for (int c = 1; c <= D.Nz; ++c) {
+      int len, nseg;
+      seg_shape(D.Nz, c, len, nseg);
+      if (c > 1 && len < 16) break;
+      const long rounds = ((long)tiles * nseg + ncu - 1) / ncu;
+      const long cost = rounds * (len + (nseg > 1 ? 2 : 0));
+      if (best < 0 || cost < best) { best = cost; n = c; }
+    }
   }
   n = std::max(1, std::min(n, (D.Nz + 1) / 2));
-  int len = (D.Nz + n - 1) / n;
-  len += len & 1;
-  D.zseg = len;
-  D.nseg = (D.Nz + len - 1) / len;
+  seg_shape(D.Nz, n, D.zseg, D.nseg);
 }
 
 /* launch one RK stage (surface-layer pass first when the stage carries the physics) */

@@ -460,6 +460,7 @@ SEGMENT_CASES = [
     ("dry", dict(fe_py.CFG0, Nx=64, Ny=16, Nz=40), False, (2, 3, 5)),
     ("dry-emit", dict(fe_py.CFG0, Nx=64, Ny=16, Nz=20, hydroSubGridWrite=1, hydroForcingWrite=1), False, (10,)),
     ("moist", dict(fe_py.CFG2, Nx=64, Ny=16, Nz=36, d_zeta=40.0, hydroSubGridWrite=1), True, (2, 4)),
+    ("smagorinsky", dict(fe_py.CFG0, Nx=64, Ny=16, Nz=40, TKESelector=0, hydroSubGridWrite=1), False, (2, 3)),
 ]
 
 
@@ -486,7 +487,8 @@ def test_z_segments_are_bitwise_identical(name, params, moist, segs, tmp_path):
             for st in (0, 2):
                 m.build_frhs(st)
                 out[f"F{st}"] = m.array("hydroFldsFrhs").copy()
-                out[f"E{st}"] = m.array("sgstkeScalarsFrhs").copy()
+                if m.array("sgstkeScalarsFrhs") is not None:
+                    out[f"E{st}"] = m.array("sgstkeScalarsFrhs").copy()
                 if moist:
                     out[f"M{st}"] = m.array("moistScalarsFrhs").copy()
             assert m.L.fe_upload_state() == 0

@@ -1,7 +1,7 @@
 """Horizontal-mean profile statistics of a model state (numpy, FP64).
 
 TEST INFRASTRUCTURE ONLY: the checker for the device reduction
-`fe_profiles` and the judge of long runs.  Restates the reference's
+`fe_profiles` and the measure on which long runs are compared.  Restates the reference's
 post-processing of an output state, tutorials/notebooks/fetutorialfunctions.py:
 compute_mean_profiles (:55-83: level means of u, v, theta) and
 compute_turb_profiles (:344-431: perturbations from the instantaneous level

@@ -252,10 +252,11 @@ static void seg_shape(int Nz, int n, int &len, int &nseg) {
 }
 
 /* A work-group holds a compute unit (its LDS), so a launch runs in rounds of one work-group
- * per CU; a segment also computes 2 warm-up levels below its first stored level.  The count
- * minimises rounds x (segment length + 2) over segments of at least 16 levels (fewest
- * segments on ties): a count whose work-groups overflow the CUs by a few would add a nearly
- * empty round. */
+ * per CU; a segment also computes 2 warm-up levels below its first stored level and fills its
+ * pipeline (about 2 levels' worth of set-up).  The count minimises
+ * rounds x (segment length + 4) over segments of at least 4 levels (fewest segments on ties):
+ * a count whose work-groups overflow the CUs by a few would add a nearly empty round, and a
+ * small grid gets enough work-groups to spread over the CUs. */
 void choose_segments() {
   const int tiles = ((D.Nx + TX - 1) / TX) * ((D.Ny + TY - 1) / TY);
   static int ncu = 0;
@@ -272,9 +273,9 @@ void choose_segments() {
     for (int c = 1; c <= D.Nz; ++c) {
       int len, nseg;
       seg_shape(D.Nz, c, len, nseg);
-      if (c > 1 && len < 16) break;
+      if (c > 1 && len < 4) break;
       const long rounds = ((long)tiles * nseg + ncu - 1) / ncu;
-      const long cost = rounds * (len + (nseg > 1 ? 2 : 0));
+      const long cost = rounds * (len + (nseg > 1 ? 4 : 0));
       if (best < 0 || cost < best) { best = cost; n = c; }
     }
   }

@@ -102,7 +102,7 @@ void *fe_host_array(const char *name, long *count) {
     {"xPos", xPos, n3}, {"yPos", yPos, n3}, {"zPos", zPos, n3}, {"J31", J31, n3}, {"J32", J32, n3}, {"J33", J33, n3},
     {"D_Jac", D_Jac, n3}, {"invD_Jac", invD_Jac, n3},
     {"cdFld", cdFld, n2}, {"chFld", chFld, n2}, {"cqFld", cqFld, n2}, {"fricVel", fricVel, n2}, {"htFlux", htFlux, n2},
-    {"tskin", tskin, n2}, {"invOblen", invOblen, n2}, {"z0m", z0m, n2}, {"z0t", z0t, n2}, {"qFlux", qFlux, n2}, {"qskin", qskin, n2},
+    {"tskin", tskin, n2}, {"topoPos", topoPos, n2}, {"invOblen", invOblen, n2}, {"z0m", z0m, n2}, {"z0t", z0t, n2}, {"qFlux", qFlux, n2}, {"qskin", qskin, n2},
   };
   for (size_t i = 0; i < sizeof t / sizeof t[0]; i++)
     if (strcmp(t[i].nm, name) == 0) {

@@ -7,8 +7,10 @@
  * (:386-410, zDeform :880-892), periodic single-rank coordinate halos
  * (:816-877), boundary/ground/ceiling coordinate halos (:480-562) and the
  * metric tensor by second-order centred differences with halo copies
- * (:584-741).  Terrain files, grid files and restart input need netCDF or a
- * topography reader and are rejected with an error.
+ * (:584-741).  A topoFile (raw binary: int Nx, int Ny, then Nx*Ny floats with x
+ * fastest, :299-347) is read and each column compressed terrain-following
+ * (:406), with the halo columns taking the nearest interior elevation
+ * (:355-383).  Grid files need netCDF and are rejected with an error.
  */
 #include <limits.h>
 #include <float.h>
@@ -109,8 +111,8 @@ int gridInit(void) {
     printf("ERROR: Nh = %d; the 5th-order advection stencil needs Nh >= 3\n", Nh);
     return -1;
   }
-  if (gridFile != NULL || topoFile != NULL) {
-    printf("ERROR: gridFile/topoFile input is not supported by this build (flat terrain only)\n");
+  if (gridFile != NULL) {
+    printf("ERROR: gridFile input (netCDF) is not supported by this build\n");
     return -1;
   }
   dX = d_xi; dY = d_eta; dZ = d_zeta;
@@ -148,7 +150,40 @@ static float zDeform(float zRect, float zGround, float zCeiling) {
 int gridSecondaryPreparations(void) {
   int i, j, k;
   const long sI = (long)(Nyp + 2 * Nh) * (Nzp + 2 * Nh), sJ = Nzp + 2 * Nh;
-  /* flat topography (grid.c:342-352): topoPos = 0 */
+  /* topography (grid.c:299-383): read from topoFile, else 0; halo columns take the
+   * nearest interior elevation */
+  if (topoFile != NULL) {
+    FILE *fp = fopen(topoFile, "rb");
+    if (!fp) {
+      printf("ERROR: cannot open topoFile %s\n", topoFile);
+      return -1;
+    }
+    int tnx = 0, tny = 0;
+    float *tin = NULL;
+    int ok = fread(&tnx, sizeof(int), 1, fp) == 1 && fread(&tny, sizeof(int), 1, fp) == 1;
+    if (ok && (tnx != Nx || tny != Ny)) {
+      printf("ERROR: topoFile extents (%d,%d) != (Nx,Ny) = (%d,%d)\n", tnx, tny, Nx, Ny);
+      ok = 0;
+    }
+    if (ok) {
+      tin = (float *)malloc(sizeof(float) * (size_t)Nx * Ny);
+      ok = tin && fread(tin, sizeof(float), (size_t)Nx * Ny, fp) == (size_t)Nx * Ny;
+    }
+    fclose(fp);
+    if (!ok) {
+      free(tin);
+      printf("ERROR: reading topoFile %s failed\n", topoFile);
+      return -1;
+    }
+    for (i = iMin - Nh; i < iMax + Nh; i++)
+      for (j = jMin - Nh; j < jMax + Nh; j++) {
+        int ig = rankXid * Nxp + i - Nh, jg = rankYid * Nyp + j - Nh;
+        ig = ig < 0 ? 0 : (ig > Nx - 1 ? Nx - 1 : ig);
+        jg = jg < 0 ? 0 : (jg > Ny - 1 ? Ny - 1 : jg);
+        topoPos[(long)i * (Nyp + 2 * Nh) + j] = tin[(long)jg * Nx + ig];
+      }
+    free(tin);
+  }
   /* interior coordinates (grid.c:386-410) */
   float ztop = ((float)Nz) * d_zeta - 0.5 * d_zeta;
   float zbot = 0.0f;
@@ -160,7 +195,8 @@ int gridSecondaryPreparations(void) {
         yPos[n] = ((float)((rankYid)*Nyp) + (float)(j - Nh)) * d_eta + 0.5 * d_eta;
         zPos[n] = ((float)k - Nh) * d_zeta + 0.5 * d_zeta;
         if (verticalDeformSwitch == 1) zPos[n] = zDeform(zPos[n], zbot, ztop);
-        zPos[n] = zPos[n] * ((ztop - 0.0f) / ztop) + 0.0f;
+        const float h = topoPos[(long)i * (Nyp + 2 * Nh) + j];
+        zPos[n] = zPos[n] * ((ztop - h) / ztop) + h;     /* terrain-following compression */
       }
   /* single-rank periodic coordinate halos (grid.c:816-877) */
   if (coordHorizHalos == 1) {

@@ -299,8 +299,8 @@ def test_long_run_profiles_dry_cbl(tmp_path):
         m.close()
 
 
-@pytest.mark.parametrize("moist", [False, True])
-def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
+@pytest.mark.parametrize("moist,tke", [(False, True), (True, True), (False, False)], ids=["dry", "moist", "smagorinsky"])
+def test_batch_end_emission_of_stresses_and_tendencies(moist, tke, tmp_path):
     """hydroSubGridWrite = 1 and hydroForcingWrite = 1: after a cuda_timeIntCommence batch the host
     holds the SGS stresses (hydroTauFlds, moistTauFlds) and the tendencies (hydroFldsFrhs, ...)
     of the batch's last stage, as the reference's device-to-host copy leaves them
@@ -310,6 +310,8 @@ def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
     else:
         p = dict(fe_py.CFG0, Nx=64, Ny=16, Nz=24)
     p.update(NtBatch=2, hydroSubGridWrite=1, hydroForcingWrite=1)
+    if not tke:
+        p.update(TKESelector=0)
     m = fe_py.Model(p, str(tmp_path))
     try:
         fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=31)
@@ -322,7 +324,7 @@ def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
         m.commence()
         o32.steps(2)
         o64.steps(2)
-        pairs = [("hydroTauFlds", "TAU", 9), ("hydroFldsFrhs", "HYDRO_F", 5), ("sgstkeScalarsFrhs", "TKE_F", 1)]
+        pairs = [("hydroTauFlds", "TAU", 9), ("hydroFldsFrhs", "HYDRO_F", 5)] + ([("sgstkeScalarsFrhs", "TKE_F", 1)] if tke else [])
         if moist:
             pairs += [("moistTauFlds", "MOISTTAU", 6), ("moistScalarsFrhs", "MOIST_F", 2)]
         for hname, oname, nf in pairs:
@@ -332,7 +334,21 @@ def test_batch_end_emission_of_stresses_and_tendencies(moist, tmp_path):
             e_gpu, e_ref = rel_err(g, r64), rel_err(r32, r64)
             print(f"emitted {hname}: gpu-vs-fp64 {e_gpu}  ref32-vs-fp64 {e_ref}")
             assert np.all(np.isfinite(g))
-            assert np.all(e_gpu <= np.maximum(1e-4, 2 * e_ref)), (hname, e_gpu, e_ref)
+            if tke or hname != "hydroTauFlds":
+                assert np.all(e_gpu <= np.maximum(1e-4, 2 * e_ref)), (hname, e_gpu, e_ref)
+            else:
+                # Smagorinsky K_m = (c_s Delta)^2 sqrt(max(0, S^2 - 3 N^2)) is ill-conditioned
+                # where the shear and buoyancy terms nearly cancel: there a rounding difference
+                # in the state moves K_m by a large fraction, and which cells sit near the cut-off
+                # differs between any two evaluations.  The stresses are held to the FP32
+                # reference's own deviation in the RMS, and their maximum to 10x its maximum.
+                ax = tuple(range(1, g.ndim))
+                sc = np.maximum(np.abs(r64).max(axis=ax), 1e-300)
+                rms_gpu = np.sqrt(((g - r64) ** 2).mean(axis=ax)) / sc
+                rms_ref = np.sqrt(((r32 - r64) ** 2).mean(axis=ax)) / sc
+                print(f"emitted {hname} rms: gpu-vs-fp64 {rms_gpu}  ref32-vs-fp64 {rms_ref}")
+                assert np.all(rms_gpu <= 2 * rms_ref + 1e-7), (hname, rms_gpu, rms_ref)
+                assert np.all(e_gpu <= np.maximum(1e-4, 10 * e_ref)), (hname, e_gpu, e_ref)
         # profile statistics with the SGS stress columns: device reduction against the numpy
         # restatement over the same downloaded state and emitted stresses
         import fe_profiles

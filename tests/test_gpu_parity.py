@@ -420,6 +420,10 @@ OPTIONS = [
     ("moist without SGS moisture fluxes", dict(moistureSGSturb=0), True),
     ("Smagorinsky closure", dict(TKESelector=0), False),
     ("moist Smagorinsky closure", dict(TKESelector=0), True),
+    ("moist stretched grid with filter", dict(verticalDeformSwitch=1, verticalDeformFactor=0.7, filterSelector=1,
+                                               filter_6th_coeff=0.03), True),
+    ("dry stretched grid with quadratic deformation", dict(verticalDeformSwitch=1, verticalDeformFactor=0.6,
+                                                            verticalDeformQuadCoeff=0.5), False),
 ]
 
 

@@ -424,6 +424,8 @@ OPTIONS = [
                                                filter_6th_coeff=0.03), True),
     ("dry stretched grid with quadratic deformation", dict(verticalDeformSwitch=1, verticalDeformFactor=0.6,
                                                             verticalDeformQuadCoeff=0.5), False),
+    ("one moisture species", dict(moistureNvars=1), True),
+    ("moist physics every stage", dict(physics_oneRKonly=0), True),
 ]
 
 

@@ -299,6 +299,49 @@ def test_long_run_profiles_dry_cbl(tmp_path):
         m.close()
 
 
+def test_long_run_profiles_moist(tmp_path):
+    """The moist shallow-cumulus physics on 64x64x48 cells (dx = 25 m, dz = 40 m), 100 RK3 steps
+    of dt = 0.05 s from the case's initial state with its moisture profile (q_v 17 g/kg at the
+    ground, 13 g/kg at 1.5 km, seeded +/-0.025 g/kg below 1.6 km), on the GPU and on the FP64
+    and FP32 oracles.  Horizontal-mean profiles including q_v and q_l must agree with FP64 at
+    least as closely as the FP32 reference does, to a factor 2, or within 1e-6 of each
+    column's range."""
+    import fe_profiles
+    p = dict(fe_py.CFG2, Nx=64, Ny=64, Nz=48, d_zeta=40.0, NtBatch=100)
+    m = fe_py.Model(p, str(tmp_path))
+    try:
+        rng = np.random.default_rng(2024)
+        hydro, mq, zpos = m.array("hydroFlds"), m.array("moistScalars"), m.array("zPos")
+        sl = (slice(NH, -NH),) * 3
+        z = zpos[sl].astype(np.float64)
+        qv = np.interp(z, [0.0, 1500.0, 3000.0, 1e6], [17.0, 13.0, 4.0, 4.0])
+        qv += np.where(z < 1600.0, rng.uniform(-0.025, 0.025, size=qv.shape), 0.0)
+        mq[0][sl] = (hydro[0][sl].astype(np.float64) * qv).astype(mq.dtype)
+        mq[1][sl] = 0.0
+        o64 = oracle_py.Oracle(p, 64, threads=16)
+        o32 = oracle_py.Oracle(p, 32, threads=16)
+        o64.load_from_model(m)
+        o32.load_from_model(m)
+        m.L.fe_set_int(b"simTime_itRestart", 0)
+        m.commence()
+        o64.steps(100)
+        o32.steps(100)
+        assert m.geti("simTime_it") == 100
+        pg = fe_profiles.profiles(m.array("hydroFlds"), m.array("sgstkeScalars"), m.array("moistScalars"))
+        p64 = fe_profiles.profiles(o64.arr("HYDRO"), o64.arr("TKE")[:1], o64.arr("MOIST")[:2])
+        p32 = fe_profiles.profiles(o32.arr("HYDRO"), o32.arr("TKE")[:1], o32.arr("MOIST")[:2])
+        np.testing.assert_allclose(m.profiles(), pg, rtol=1e-9, atol=1e-12)
+        rng_ = np.maximum(np.abs(p64).max(axis=0), 1e-30)
+        e_gpu = (np.abs(pg - p64)).max(axis=0) / rng_
+        e_ref = (np.abs(p32 - p64)).max(axis=0) / rng_
+        print(f"100-step moist profiles: gpu-vs-fp64 {e_gpu}\n                         ref32-vs-fp64 {e_ref}")
+        assert np.all(e_gpu <= np.maximum(2 * e_ref, 1e-6)), (e_gpu, e_ref)
+        o64.close()
+        o32.close()
+    finally:
+        m.close()
+
+
 @pytest.mark.parametrize("moist,tke", [(False, True), (True, True), (False, False)], ids=["dry", "moist", "smagorinsky"])
 def test_batch_end_emission_of_stresses_and_tendencies(moist, tke, tmp_path):
     """hydroSubGridWrite = 1 and hydroForcingWrite = 1: after a cuda_timeIntCommence batch the host

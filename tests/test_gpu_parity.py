@@ -635,3 +635,24 @@ def test_large_scale_forcing_matches_oracle(moist, tmp_path):
         ono.close()
     finally:
         m.close()
+
+
+SEGMENT_CHOICE = [
+    ((64, 64, 48), "12 z-segments of 4 levels"),      # 8 tiles: spread over the CUs
+    ((152, 146, 122), "4 z-segments of 32 levels"),   # 57 tiles: 5 segments would overflow 256 CUs
+    ((256, 256, 160), "2 z-segments of 80 levels"),   # 128 tiles
+    ((512, 64, 128), "4 z-segments of 32 levels"),    # 64 tiles x 4 = 256: one round of 32 + 4 levels
+]
+
+
+@pytest.mark.parametrize("shape,expect", SEGMENT_CHOICE, ids=["x".join(map(str, c[0])) for c in SEGMENT_CHOICE])
+def test_segment_count_choice(shape, expect, tmp_path):
+    """The z-segment count minimises launch rounds of one work-group per CU (256 on MI355X)
+    times (segment length + 4); the kernel info names the choice."""
+    nx, ny, nz = shape
+    m = fe_py.Model(dict(fe_py.CFG0, Nx=nx, Ny=ny, Nz=nz, NtBatch=1), str(tmp_path))
+    try:
+        info = m.L.fe_kernel_info().decode()
+        assert info.endswith(expect), info
+    finally:
+        m.close()

@@ -163,14 +163,9 @@ struct Cfg {
   }
 };
 
-#ifndef FE_LDSMODE
-#define FE_LDSMODE 0
-#endif
-#if FE_LDSMODE == 1
-typedef volatile __attribute__((address_space(3))) float SRD;
-#else
+/* LDS-qualified float: stencil reads through it compile to ds_read (paired where the offsets
+ * allow) rather than flat loads whose address space the compiler would have to prove */
 typedef __attribute__((address_space(3))) float SRD;
-#endif
 template <class C, int F, int FE = C::NF>
 struct FaceLoop {
   __device__ __forceinline__ static void xf(float (*raw)[RH][RW], float (*gx)[TY][TX + 1], int fy, int fx, float U, const float *omb) {

@@ -12,7 +12,7 @@ device starts from (float32), and the interior tendencies of RK stages 0 and 2
 from the FP64 oracle (float64) and the FP32 oracle (float32).
 
 Run from the repository root after building the library and the oracle:
-    python3 tests/golden/make_golden.py
+    python3 tests/golden/make_golden.py [case ...]
 """
 import json
 import os
@@ -30,6 +30,7 @@ import oracle_py  # noqa: E402
 CASES = {
     "dry_12x10x12": (dict(fe_py.CFG0, Nx=12, Ny=10, Nz=12), 7, False),
     "moist_10x8x10": (dict(fe_py.CFG2, Nx=10, Ny=8, Nz=10, d_zeta=120.0), 8, True),
+    "smagorinsky_12x10x12": (dict(fe_py.CFG0, Nx=12, Ny=10, Nz=12, TKESelector=0), 9, False),
 }
 FIELDS = (("hydroFlds", "HYDRO", "hydroFldsFrhs", "HYDRO_F"), ("sgstkeScalars", "TKE", "sgstkeScalarsFrhs", "TKE_F"),
           ("moistScalars", "MOIST", "moistScalarsFrhs", "MOIST_F"))
@@ -68,7 +69,9 @@ def run_case(params, seed, moist):
 
 
 def main():
-    for name, (params, seed, moist) in CASES.items():
+    names = sys.argv[1:] or list(CASES)            # regenerate the named cases only, if any given
+    for name in names:
+        params, seed, moist = CASES[name]
         out = run_case(params, seed, moist)
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)

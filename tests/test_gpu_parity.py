@@ -196,7 +196,7 @@ def test_rk3_steps_moist(tmp_path):
         m.close()
 
 
-@pytest.mark.parametrize("name", ["dry_12x10x12", "moist_10x8x10"])
+@pytest.mark.parametrize("name", ["dry_12x10x12", "moist_10x8x10", "smagorinsky_12x10x12"])
 def test_stage_tendencies_against_golden(name, tmp_path):
     """The committed oracle fixtures: load the stored state, build the stage-0 and stage-2
     tendencies on the GPU and compare with the stored FP64/FP32 oracle tendencies."""

@@ -590,8 +590,9 @@ LSF_STRONG = dict(lsfSelector=1, lsf_freq=0.05,
                   lsf_qv_surf=-150.0, lsf_qv_lev1=-100.0, lsf_qv_lev2=50.0, lsf_qv_zlev1=100.0, lsf_qv_zlev2=250.0)
 
 
-@pytest.mark.parametrize("moist", [True, False], ids=["moist_subsidence", "dry"])
-def test_large_scale_forcing_matches_oracle(moist, tmp_path):
+@pytest.mark.parametrize("moist,stretched", [(True, False), (False, False), (True, True)],
+                         ids=["moist_subsidence", "dry", "moist_subsidence_stretched"])
+def test_large_scale_forcing_matches_oracle(moist, stretched, tmp_path):
     """Large-scale forcing (lsfSelector = 1; with the subsidence terms of the slab means in the
     moist case) through three RK3 steps against the FP64 oracle, with the forcing due every
     step after the first (lsf_freq = dt).  The forcing is made strong enough that the oracle's
@@ -601,6 +602,8 @@ def test_large_scale_forcing_matches_oracle(moist, tmp_path):
         p = dict(fe_py.CFG2, Nx=32, Ny=16, Nz=16, d_zeta=25.0, dt=0.05, NtBatch=1, lsf_horMnSubTerms=1, **LSF_STRONG)
     else:
         p = dict(fe_py.CFG0, Nx=48, Ny=16, Nz=20, dt=0.05, NtBatch=1, lsf_horMnSubTerms=0, **LSF_STRONG)
+    if stretched:
+        p.update(verticalDeformSwitch=1, verticalDeformFactor=0.7)
     m = fe_py.Model(p, str(tmp_path))
     try:
         fe_py.turbulent_state((m.array("hydroFlds"), m.array("sgstkeScalars")), seed=17)
